@@ -1,0 +1,102 @@
+"""ctypes binding of libemerge_amd.so (the C-ABI declared in include/emerge_amd.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemerge_amd.so")
+
+EM_OK = 0
+EM_EINVAL = -1
+EM_EUNSUPPORTED = -2
+EM_EHIP = -3
+EM_ENOMEM = -4
+EM_EDEGENERATE = -5
+EM_ETABLES = -6
+
+EM_MAT_SCALAR = 0
+EM_MAT_DIAG = 1
+EM_MAT_FULL = 2
+EM_MAT_IS_INVERSE = 1
+
+EM_PREC_FP64 = 0
+EM_PREC_MIXED = 1
+
+
+class EmError(RuntimeError):
+    """A non-zero status returned by the C-ABI, with ``em_last_error()``'s message."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"emerge_amd status {status}: {msg}")
+        self.status = status
+
+
+class DegenerateTetError(EmError):
+    pass
+
+
+class UnsupportedError(EmError):
+    pass
+
+
+class TableError(EmError):
+    pass
+
+
+class em_material_t(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("kind", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("tet_stride", ctypes.c_int64), ("comp_stride", ctypes.c_int64)]
+
+
+_lib = None
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+
+_SIGS = {
+    "em_last_error": (ctypes.c_char_p, []),
+    "em_version": (ctypes.c_char_p, []),
+    "em_ned2_plan_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_int, P, I64, P, I64, P, I64,
+                                           P, I64, P, P, P, I64, P, I64, P, I32]),
+    "em_ned2_plan_destroy": (ctypes.c_int, [P]),
+    "em_ned2_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64),
+                                          ctypes.POINTER(I64)]),
+    "em_ned2_assemble_km": (ctypes.c_int, [P, ctypes.POINTER(em_material_t),
+                                           ctypes.POINTER(em_material_t), P, P, I32, P]),
+    "em_ned2_assemble_a": (ctypes.c_int, [P, ctypes.POINTER(em_material_t),
+                                          ctypes.POINTER(em_material_t), P, I32, P, I64, I32, P]),
+    "em_ned2_rowcol": (ctypes.c_int, [P, P, P, P]),
+}
+
+# every symbol include/emerge_amd.h declares
+EXPORTED = tuple(_SIGS)
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m emerge_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status == EM_OK:
+        return
+    msg = lib().em_last_error().decode(errors="replace")
+    cls = {EM_EDEGENERATE: DegenerateTetError, EM_EUNSUPPORTED: UnsupportedError,
+           EM_ETABLES: TableError}.get(status, EmError)
+    raise cls(status, msg)

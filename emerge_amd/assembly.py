@@ -1,0 +1,234 @@
+"""Host side of the Nedelec-2 assembly: device plans and material descriptors.
+
+``Ned2Plan`` keeps one mesh resident on the GPU (through the C-ABI in
+include/emerge_amd.h) and launches the HIP kernels; torch tensors are only the device
+buffers. The reference-shaped entry point ``tet_mass_stiffness_matrices`` lives in
+``emerge_amd.curlcurl``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, em_material_t
+
+PREC_FP64 = _lib.EM_PREC_FP64
+PREC_MIXED = _lib.EM_PREC_MIXED
+_PREC = {"fp64": PREC_FP64, "mixed": PREC_MIXED, PREC_FP64: PREC_FP64, PREC_MIXED: PREC_MIXED}
+
+
+def _prec(p) -> int:
+    try:
+        return _PREC[p]
+    except KeyError:
+        raise ValueError(f"unknown precision {p!r}; use 'fp64' or 'mixed'") from None
+
+
+def _stream_ptr(stream) -> ctypes.c_void_p:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a), dtype=np.int64)
+
+
+@dataclass
+class Material:
+    """A complex128 device tensor field and how to index it (see ``em_material_t``).
+
+    ``tensor`` keeps the device storage alive for as long as the descriptor is used.
+    """
+    tensor: torch.Tensor
+    kind: int
+    tet_stride: int
+    comp_stride: int
+    flags: int = 0
+
+    def c_struct(self) -> em_material_t:
+        return em_material_t(ctypes.c_void_p(self.tensor.data_ptr()), self.kind, self.flags,
+                             self.tet_stride, self.comp_stride)
+
+
+def material(value, n_tets: int, device, inverse: bool = False, detect: bool = True,
+             layout: str | None = None) -> Material:
+    """Describe a per-tet material tensor field for the kernels.
+
+    Layouts: ``"ref"`` is the reference's ``(3, 3, T)`` array (``assembler.py:440-453``);
+    ``"tet"`` is tet-major: ``(T,)`` scalar per tet, ``(T, 3)`` diagonal, ``(T, 3, 3)``
+    full. A Python/NumPy scalar is one value for every tet. By default NumPy arrays of
+    shape ``(3, 3, T)`` are ``"ref"`` and everything else is ``"tet"``.
+    For a ``"ref"`` host array with ``detect=True`` the structure is checked on the host
+    and a diagonal or scalar field is described as such (its diagonal is read in place).
+    ``inverse`` marks a mu field that already holds mu_r^-1.
+    """
+    flags = _lib.EM_MAT_IS_INVERSE if inverse else 0
+    if np.isscalar(value) or (isinstance(value, (np.ndarray, torch.Tensor)) and value.ndim == 0):
+        t = torch.tensor([complex(value)], dtype=torch.complex128, device=device)
+        return Material(t, _lib.EM_MAT_SCALAR, 0, 0, flags)
+    is_torch = isinstance(value, torch.Tensor)
+    shape = tuple(value.shape)
+    if layout is None:
+        layout = "ref" if (not is_torch and len(shape) == 3 and shape[:2] == (3, 3)) else "tet"
+    if is_torch:
+        t = value.to(device=device, dtype=torch.complex128).contiguous()
+    else:
+        arr = np.ascontiguousarray(value, dtype=np.complex128)
+        t = torch.from_numpy(arr).to(device)
+    if layout == "ref":
+        if shape != (3, 3, n_tets):
+            raise ValueError(f"reference-layout material must be (3, 3, {n_tets}), got {shape}")
+        kind = _lib.EM_MAT_FULL
+        if detect and not is_torch:
+            off = arr[[0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1], :]
+            if not np.any(off):
+                kind = _lib.EM_MAT_DIAG
+                if np.array_equal(arr[0, 0], arr[1, 1]) and np.array_equal(arr[0, 0], arr[2, 2]):
+                    kind = _lib.EM_MAT_SCALAR
+        if kind == _lib.EM_MAT_FULL:
+            return Material(t, kind, 1, n_tets, flags)
+        if kind == _lib.EM_MAT_DIAG:
+            return Material(t, kind, 1, 4 * n_tets, flags)
+        return Material(t, kind, 1, 0, flags)
+    if shape == (n_tets,):
+        return Material(t, _lib.EM_MAT_SCALAR, 1, 0, flags)
+    if shape == (n_tets, 3):
+        return Material(t, _lib.EM_MAT_DIAG, 3, 1, flags)
+    if shape == (n_tets, 3, 3):
+        return Material(t, _lib.EM_MAT_FULL, 9, 1, flags)
+    raise ValueError(f"material of shape {shape} does not match {n_tets} tets")
+
+
+class Ned2Plan:
+    """A mesh resident on one GPU, ready for repeated assembly.
+
+    Parameters follow the reference tables (``Mesh3D``/``Nedelec2`` attributes):
+    ``nodes (3,Nn)``, ``tets (4,T)``, ``tris (3,F)``, ``edges (2,E)``,
+    ``tet_to_field (20,T)``, ``tet_to_edge (6,T)``, ``tet_to_tri (4,T)``, ``n_field``,
+    optional ``tetids`` (the assembled tets, conductor tets removed) and ``dofcodes``.
+    """
+
+    def __init__(self, nodes, tets, tris, edges, tet_to_field, tet_to_edge, tet_to_tri,
+                 n_field: int, tetids=None, dofcodes=None, device=None):
+        L = _lib.lib()
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("Ned2Plan needs a GPU device")
+        nodes = np.ascontiguousarray(np.asarray(nodes), dtype=np.float64)
+        tets = _i64(tets)
+        tris, edges = _i64(tris), _i64(edges)
+        t2f, t2e, t2t = _i64(tet_to_field), _i64(tet_to_edge), _i64(tet_to_tri)
+        if dofcodes is None:
+            from .mesh import DOFCODES_NED2
+            dofcodes = DOFCODES_NED2
+        dofcodes = _i64(dofcodes)
+        self.n_tets = int(tets.shape[1])
+        if tetids is not None:
+            tetids = _i64(tetids)
+            n_assy = int(tetids.shape[0])
+        else:
+            n_assy = self.n_tets
+        self._keep = (nodes, tets, tris, edges, t2f, t2e, t2t, tetids, dofcodes)
+        h = ctypes.c_void_p()
+        ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(L.em_ned2_plan_create(ctypes.byref(h), self.device.index or 0, ptr(nodes),
+                                    nodes.shape[1], ptr(tets), self.n_tets, ptr(tris),
+                                    tris.shape[1], ptr(edges), edges.shape[1], ptr(t2f), ptr(t2e),
+                                    ptr(t2t), int(n_field), ptr(tetids), n_assy, ptr(dofcodes),
+                                    int(dofcodes.shape[0])))
+        self._h = h
+        self._keep = None
+        self.n_assy = n_assy
+        self.n_field = int(n_field)
+        self.nnz_coo = 400 * n_assy
+
+    @classmethod
+    def from_field(cls, field, tetids=None, device=None) -> "Ned2Plan":
+        m = field.mesh
+        return cls(m.nodes, m.tets, m.tris, m.edges, field.get_tet_to_field(), m.tet_to_edge,
+                   m.tet_to_tri, field.n_field, tetids=tetids, dofcodes=field.dofcodes3d,
+                   device=device)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().em_ned2_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- materials ------------------------------------------------------------
+    def material(self, value, inverse=False, detect=True, layout=None) -> Material:
+        if isinstance(value, Material):
+            return value
+        return material(value, self.n_tets, self.device, inverse=inverse, detect=detect,
+                        layout=layout)
+
+    # -- launches -------------------------------------------------------------
+    def assemble_km(self, er, ur, precision="fp64", out=None, stream=None):
+        """K (stiffness, mu_r^-1) and M (mass, eps_r) COO value arrays, reference layout."""
+        prec = _prec(precision)
+        er_m, ur_m = self.material(er), self.material(ur)
+        dt = torch.complex128 if prec == PREC_FP64 else torch.complex64
+        if out is None:
+            K = torch.empty(self.nnz_coo, dtype=dt, device=self.device)
+            M = torch.empty(self.nnz_coo, dtype=dt, device=self.device)
+        else:
+            K, M = out
+            _check_out(K, self.nnz_coo, dt)
+            _check_out(M, self.nnz_coo, dt)
+        e, u = er_m.c_struct(), ur_m.c_struct()
+        check(_lib.lib().em_ned2_assemble_km(self._h, ctypes.byref(e), ctypes.byref(u),
+                                             ctypes.c_void_p(K.data_ptr()),
+                                             ctypes.c_void_p(M.data_ptr()), prec,
+                                             _stream_ptr(stream)))
+        return K, M
+
+    def assemble_a(self, er, ur, k0, precision="fp64", out=None, stream=None):
+        """COO values of A = K - k0^2 M. ``k0`` a float (returns ``[nnz_coo]``) or a
+        sequence of wavenumbers (returns ``[len(k0), nnz_coo]``, one launch)."""
+        prec = _prec(precision)
+        er_m, ur_m = self.material(er), self.material(ur)
+        dt = torch.complex128 if prec == PREC_FP64 else torch.complex64
+        scalar = np.isscalar(k0)
+        k0a = np.ascontiguousarray(np.atleast_1d(np.asarray(k0, dtype=np.float64)))
+        nf = int(k0a.shape[0])
+        if out is None:
+            out = torch.empty((nf, self.nnz_coo), dtype=dt, device=self.device)
+        else:
+            _check_out(out, nf * self.nnz_coo, dt)
+        stride = self.nnz_coo if out.dim() == 1 else out.stride(0)
+        e, u = er_m.c_struct(), ur_m.c_struct()
+        check(_lib.lib().em_ned2_assemble_a(self._h, ctypes.byref(e), ctypes.byref(u),
+                                            k0a.ctypes.data_as(ctypes.c_void_p), nf,
+                                            ctypes.c_void_p(out.data_ptr()), stride, prec,
+                                            _stream_ptr(stream)))
+        if scalar and out.dim() == 2:
+            return out[0]
+        return out
+
+    def rowcol(self, stream=None):
+        rows = torch.empty(self.nnz_coo, dtype=torch.int64, device=self.device)
+        cols = torch.empty(self.nnz_coo, dtype=torch.int64, device=self.device)
+        check(_lib.lib().em_ned2_rowcol(self._h, ctypes.c_void_p(rows.data_ptr()),
+                                        ctypes.c_void_p(cols.data_ptr()), _stream_ptr(stream)))
+        return rows, cols
+
+
+def _check_out(t: torch.Tensor, numel: int, dt) -> None:
+    if t.dtype != dt or not t.is_cuda or t.numel() != numel:
+        raise ValueError(f"output must be a {dt} CUDA tensor with {numel} elements")
+    if t.dim() == 1 and not t.is_contiguous():
+        raise ValueError("output must be contiguous")
+    if t.dim() == 2 and t.stride(1) != 1:
+        raise ValueError("output rows must be contiguous")

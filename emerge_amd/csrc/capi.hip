@@ -1,0 +1,417 @@
+// C-ABI of libemerge_amd.so (declared in include/emerge_amd.h).
+//
+// A plan owns the device copy of one mesh: padded node coordinates, int32 tet node
+// ids, the int32 DOF map and the assembly compaction. It replaces the table arguments
+// that the reference passes to _matrix_builder on every call
+// (src/emerge/_emerge/physics/microwave/assembly/curlcurl.py:387-417).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "em_internal.h"
+
+namespace em {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+}  // namespace em
+
+using em::set_error;
+
+struct em_ned2_plan_s {
+    int device = 0;
+    long long n_nodes = 0, n_tets = 0, n_assy = 0, n_field = 0;
+    double4* xyz = nullptr;
+    int4* tets = nullptr;
+    int* t2f = nullptr;
+    int* tetids = nullptr;
+    double* k0sq = nullptr;  // device scratch for wavenumbers
+    int k0cap = 0;
+
+    em::MeshDev dev() const {
+        em::MeshDev m;
+        m.xyz = xyz;
+        m.tets = tets;
+        m.t2f = t2f;
+        m.tetids = tetids;
+        m.n_assy = n_assy;
+        return m;
+    }
+};
+
+#define EM_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));             \
+            return e_ == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;                   \
+        }                                                                             \
+    } while (0)
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(dev) == hipSuccess) ok = true;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+int dmalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return EM_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;
+    }
+    return EM_OK;
+}
+
+void free_plan(em_ned2_plan_s* p) {
+    if (!p) return;
+    (void)hipFree(p->xyz);
+    (void)hipFree(p->tets);
+    (void)hipFree(p->t2f);
+    (void)hipFree(p->tetids);
+    (void)hipFree(p->k0sq);
+    delete p;
+}
+
+int check_material(const em_material_t* m, long long n_tets, const char* name) {
+    if (!m || !m->data) {
+        set_error(std::string(name) + ": null material");
+        return EM_EINVAL;
+    }
+    if (m->kind < EM_MAT_SCALAR || m->kind > EM_MAT_FULL) {
+        set_error(std::string(name) + ": unknown material kind");
+        return EM_EINVAL;
+    }
+    if (m->tet_stride < 0 || m->comp_stride < 0) {
+        set_error(std::string(name) + ": negative stride");
+        return EM_EINVAL;
+    }
+    (void)n_tets;
+    return EM_OK;
+}
+
+em::MatDesc to_desc(const em_material_t* m, bool is_mu) {
+    em::MatDesc d;
+    d.p = reinterpret_cast<const double2*>(m->data);
+    d.kind = m->kind;
+    d.invert = (is_mu && !(m->flags & EM_MAT_IS_INVERSE)) ? 1 : 0;
+    d.ts = m->tet_stride;
+    d.cs = m->comp_stride;
+    return d;
+}
+
+// A non-symmetric tensor needs the antisymmetric contraction terms; the host cannot
+// see device data cheaply, so FULL tensors always take that path.
+int needs_anti(const em_material_t* er, const em_material_t* ur) {
+    return (er->kind == EM_MAT_FULL || ur->kind == EM_MAT_FULL) ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* em_last_error(void) { return em::g_err.c_str(); }
+
+const char* em_version(void) { return "emerge_amd 0.1.0 (gfx950)"; }
+
+int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, int64_t n_nodes,
+                        const int64_t* tets, int64_t n_tets, const int64_t* tris,
+                        int64_t n_tris, const int64_t* edges, int64_t n_edges,
+                        const int64_t* tet_to_field, const int64_t* tet_to_edge,
+                        const int64_t* tet_to_tri, int64_t n_field, const int64_t* tetids,
+                        int64_t n_assy, const int64_t* dofcodes, int32_t ndof) {
+    if (!plan) {
+        set_error("plan: null output pointer");
+        return EM_EINVAL;
+    }
+    *plan = nullptr;
+    if (!nodes || !tets || !tris || !edges || !tet_to_field || !tet_to_edge || !tet_to_tri ||
+        !dofcodes || n_nodes <= 0 || n_tets < 0 || n_tris < 0 || n_edges < 0 || n_field <= 0) {
+        set_error("plan_create: null table or non-positive size");
+        return EM_EINVAL;
+    }
+    if (n_nodes >= (1ll << 31) || n_field >= (1ll << 31) || n_tets >= (1ll << 31)) {
+        set_error("plan_create: ids must fit in int32");
+        return EM_EINVAL;
+    }
+    static const int64_t kCodes[20] = {64, 64, 64, 64, 64, 64, 66, 66, 66, 66,
+                                       66, 66, 128, 128, 128, 128, 129, 129, 129, 129};
+    if (ndof != 20 || std::memcmp(dofcodes, kCodes, sizeof(kCodes)) != 0) {
+        set_error("plan_create: only the SECOND_MIXED_VOLAKIS DOF set "
+                  "[64]*6+[66]*6+[128]*4+[129]*4 is supported");
+        return EM_EUNSUPPORTED;
+    }
+    if (tetids == nullptr) n_assy = n_tets;
+    if (n_assy < 0 || n_assy > n_tets) {
+        set_error("plan_create: n_assy out of range");
+        return EM_EINVAL;
+    }
+    // host-side conversion and range checks
+    std::vector<double4> hxyz((size_t)n_nodes);
+    for (long long n = 0; n < n_nodes; ++n)
+        hxyz[n] = make_double4(nodes[n], nodes[n_nodes + n], nodes[2 * n_nodes + n], 0.0);
+    std::vector<int4> htets((size_t)n_tets);
+    for (long long t = 0; t < n_tets; ++t) {
+        long long v[4];
+        for (int k = 0; k < 4; ++k) {
+            v[k] = tets[k * n_tets + t];
+            if (v[k] < 0 || v[k] >= n_nodes) {
+                set_error("plan_create: tet node id out of range");
+                return EM_EINVAL;
+            }
+        }
+        if (v[0] == v[1] || v[0] == v[2] || v[0] == v[3] || v[1] == v[2] || v[1] == v[3] ||
+            v[2] == v[3]) {
+            set_error("plan_create: tet with repeated node");
+            return EM_EINVAL;
+        }
+        htets[t] = make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+    }
+    std::vector<int> ht2f((size_t)n_tets * 20);
+    for (long long t = 0; t < n_tets; ++t)
+        for (int d = 0; d < 20; ++d) {
+            long long f = tet_to_field[d * n_tets + t];
+            if (f < 0 || f >= n_field) {
+                set_error("plan_create: tet_to_field entry out of range");
+                return EM_EINVAL;
+            }
+            ht2f[t * 20 + d] = (int)f;
+        }
+    std::vector<int> hids;
+    if (tetids) {
+        hids.resize((size_t)n_assy);
+        for (long long i = 0; i < n_assy; ++i) {
+            if (tetids[i] < 0 || tetids[i] >= n_tets) {
+                set_error("plan_create: tetids entry out of range");
+                return EM_EINVAL;
+            }
+            hids[i] = (int)tetids[i];
+        }
+    }
+
+    DeviceGuard g(device);
+    if (!g.ok) {
+        set_error("plan_create: cannot select device");
+        return EM_EHIP;
+    }
+    auto* p = new em_ned2_plan_s();
+    p->device = device;
+    p->n_nodes = n_nodes;
+    p->n_tets = n_tets;
+    p->n_assy = n_assy;
+    p->n_field = n_field;
+    int st;
+    if ((st = dmalloc(&p->xyz, hxyz.size())) || (st = dmalloc(&p->tets, htets.size())) ||
+        (st = dmalloc(&p->t2f, ht2f.size())) || (st = dmalloc(&p->tetids, hids.size()))) {
+        free_plan(p);
+        return st;
+    }
+    auto cp = [&](void* d, const void* h, size_t bytes) -> int {
+        if (bytes == 0) return EM_OK;
+        hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+            return EM_EHIP;
+        }
+        return EM_OK;
+    };
+    if ((st = cp(p->xyz, hxyz.data(), hxyz.size() * sizeof(double4))) ||
+        (st = cp(p->tets, htets.data(), htets.size() * sizeof(int4))) ||
+        (st = cp(p->t2f, ht2f.data(), ht2f.size() * sizeof(int))) ||
+        (st = cp(p->tetids, hids.data(), hids.size() * sizeof(int)))) {
+        free_plan(p);
+        return st;
+    }
+
+    // Validate the edge/face tables and the volumes on the device.
+    long long *d_edges = nullptr, *d_tris = nullptr, *d_t2e = nullptr, *d_t2t = nullptr;
+    unsigned long long* d_bad = nullptr;
+    unsigned long long bad[2] = {0, 0};
+    int vst = EM_OK;
+    if ((vst = dmalloc(&d_edges, (size_t)2 * n_edges)) || (vst = dmalloc(&d_tris, (size_t)3 * n_tris)) ||
+        (vst = dmalloc(&d_t2e, (size_t)6 * n_tets)) || (vst = dmalloc(&d_t2t, (size_t)4 * n_tets)) ||
+        (vst = dmalloc(&d_bad, 2))) {
+    } else if ((vst = cp(d_edges, edges, sizeof(long long) * 2 * n_edges)) ||
+               (vst = cp(d_tris, tris, sizeof(long long) * 3 * n_tris)) ||
+               (vst = cp(d_t2e, tet_to_edge, sizeof(long long) * 6 * n_tets)) ||
+               (vst = cp(d_t2t, tet_to_tri, sizeof(long long) * 4 * n_tets))) {
+    } else {
+        hipError_t e = hipMemset(d_bad, 0, 2 * sizeof(unsigned long long));
+        if (e == hipSuccess)
+            e = em::launch_check_tables(p->tets, n_tets, d_edges, n_edges, d_tris, n_tris, d_t2e,
+                                        d_t2t, d_bad, nullptr);
+        if (e == hipSuccess)
+            e = em::launch_check_volumes(p->xyz, p->tets, n_tets, d_bad + 1, nullptr);
+        if (e == hipSuccess)
+            e = hipMemcpy(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            set_error(std::string("plan validation: ") + hipGetErrorString(e));
+            vst = EM_EHIP;
+        }
+    }
+    (void)hipFree(d_edges);
+    (void)hipFree(d_tris);
+    (void)hipFree(d_t2e);
+    (void)hipFree(d_t2t);
+    (void)hipFree(d_bad);
+    if (vst != EM_OK) {
+        free_plan(p);
+        return vst;
+    }
+    if (bad[0]) {
+        set_error("plan_create: " + std::to_string(bad[0]) +
+                  " edge/face slots differ from the sorted-tuple tables of the tets");
+        free_plan(p);
+        return EM_ETABLES;
+    }
+    if (bad[1]) {
+        set_error("plan_create: " + std::to_string(bad[1]) + " tets with zero volume");
+        free_plan(p);
+        return EM_EDEGENERATE;
+    }
+    *plan = p;
+    return EM_OK;
+}
+
+int em_ned2_plan_destroy(em_ned2_plan_t plan) {
+    if (!plan) return EM_OK;
+    DeviceGuard g(plan->device);
+    free_plan(plan);
+    return EM_OK;
+}
+
+int em_ned2_plan_sizes(em_ned2_plan_t plan, int64_t* n_assy, int64_t* n_field,
+                       int64_t* nnz_coo) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    if (n_assy) *n_assy = plan->n_assy;
+    if (n_field) *n_field = plan->n_field;
+    if (nnz_coo) *nnz_coo = plan->n_assy * 400;
+    return EM_OK;
+}
+
+int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er, const em_material_t* ur,
+                        void* dataE, void* dataB, int32_t precision, void* stream) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    int st;
+    if ((st = check_material(er, plan->n_tets, "er")) || (st = check_material(ur, plan->n_tets, "ur")))
+        return st;
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("unknown precision");
+        return EM_EINVAL;
+    }
+    if (plan->n_assy > 0 && (!dataE || !dataB)) {
+        set_error("null output");
+        return EM_EINVAL;
+    }
+    if (((uintptr_t)dataE & 15) || ((uintptr_t)dataB & 15)) {
+        set_error("outputs must be 16-byte aligned");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    const double zero = 0.0;
+    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_KM,
+                               precision, needs_anti(er, ur), nullptr, &zero, 0, dataE, dataB, 0,
+                               0, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er, const em_material_t* ur,
+                       const double* k0, int32_t n_k0, void* out, int64_t freq_stride,
+                       int32_t precision, void* stream) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    int st;
+    if ((st = check_material(er, plan->n_tets, "er")) || (st = check_material(ur, plan->n_tets, "ur")))
+        return st;
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("unknown precision");
+        return EM_EINVAL;
+    }
+    if (!k0 || n_k0 <= 0) {
+        set_error("assemble_a: need at least one k0");
+        return EM_EINVAL;
+    }
+    if (n_k0 > 1 && freq_stride < plan->n_assy * 400) {
+        set_error("assemble_a: freq_stride smaller than one COO array");
+        return EM_EINVAL;
+    }
+    if (plan->n_assy > 0 && !out) {
+        set_error("null output");
+        return EM_EINVAL;
+    }
+    if (((uintptr_t)out & 15) || (precision == EM_PREC_MIXED && n_k0 > 1 && (freq_stride & 1))) {
+        set_error("output must be 16-byte aligned (complex64 sweeps need an even freq_stride)");
+        return EM_EINVAL;
+    }
+    std::vector<double> k0sq((size_t)n_k0);
+    for (int f = 0; f < n_k0; ++f) {
+        if (!std::isfinite(k0[f])) {
+            set_error("assemble_a: non-finite k0");
+            return EM_EINVAL;
+        }
+        k0sq[f] = k0[f] * k0[f];
+    }
+    DeviceGuard g(plan->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (n_k0 == 1) {
+        EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_A,
+                                   precision, needs_anti(er, ur), nullptr, k0sq.data(), 1, out,
+                                   nullptr, 0, 0, s));
+        return EM_OK;
+    }
+    if (plan->k0cap < n_k0) {
+        EM_HIP(hipStreamSynchronize(s));
+        (void)hipFree(plan->k0sq);
+        plan->k0sq = nullptr;
+        plan->k0cap = 0;
+        if ((st = dmalloc(&plan->k0sq, (size_t)n_k0))) return st;
+        plan->k0cap = n_k0;
+    }
+    EM_HIP(hipMemcpyAsync(plan->k0sq, k0sq.data(), sizeof(double) * n_k0, hipMemcpyHostToDevice, s));
+    // the host vector must outlive the copy
+    EM_HIP(hipStreamSynchronize(s));
+    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_AF,
+                               precision, needs_anti(er, ur), plan->k0sq, k0sq.data(), n_k0, out,
+                               nullptr, freq_stride, 0, s));
+    return EM_OK;
+}
+
+int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stream) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    if (plan->n_assy > 0 && (!rows || !cols)) {
+        set_error("null output");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    EM_HIP(em::launch_rowcol(plan->dev(), (long long*)rows, (long long*)cols, (hipStream_t)stream));
+    return EM_OK;
+}
+
+}  // extern "C"

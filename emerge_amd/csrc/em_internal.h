@@ -1,0 +1,49 @@
+// Internal declarations shared by the HIP sources of libemerge_amd (not part of the
+// public C-ABI, which is include/emerge_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/emerge_amd.h"
+
+namespace em {
+
+// Device view of a per-tet material tensor field, complex128 interleaved.
+// value(t, component) = p[t * ts + component * cs]; ts = 0 makes it uniform.
+struct MatDesc {
+    const double2* p;
+    int kind;    // EM_MAT_SCALAR / EM_MAT_DIAG / EM_MAT_FULL
+    int invert;  // 1: the stored tensor is mu_r and must be inverted (reference matinv)
+    long long ts;
+    long long cs;
+};
+
+// Mesh data resident on the device for one plan.
+struct MeshDev {
+    const double4* xyz;     // [Nn] node coordinates, padded to 32 B
+    const int4* tets;       // [T] node ids of each tet, stored vertex order
+    const int* t2f;         // [T][20] DOF map (tet_to_field transposed, int32)
+    const int* tetids;      // [Ta] assembled tet of each assembly index, or nullptr
+    long long n_assy;
+};
+
+enum OutMode { MODE_A = 0, MODE_KM = 1, MODE_AF = 2 };
+
+// Launchers (defined in ned2_assemble.hip). All return hipError_t of the launch.
+hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
+                           int mode, int precision, int anti, const double* k0sq_dev,
+                           const double* k0sq_host, int nf, void* out0, void* out1,
+                           long long freq_stride, int nb, hipStream_t stream);
+hipError_t launch_rowcol(const MeshDev& m, long long* rows, long long* cols, hipStream_t s);
+hipError_t launch_check_tables(const int4* tets, long long T, const long long* edges,
+                               long long E, const long long* tris, long long F,
+                               const long long* t2e, const long long* t2t,
+                               unsigned long long* bad, hipStream_t s);
+hipError_t launch_check_volumes(const double4* xyz, const int4* tets, long long T,
+                                unsigned long long* bad, hipStream_t s);
+
+void set_error(const std::string& msg);
+
+}  // namespace em

@@ -1,0 +1,548 @@
+// Nedelec-2 tetrahedral curl-curl assembly kernels for gfx950.
+//
+// Reference: src/emerge/_emerge/physics/microwave/assembly/curlcurl.py:285-456
+// (ned2_tet_stiff_mass + _matrix_builder). The reference integrates every element with
+// an 11-point rule in physical coordinates (~1.2e5 flop/tet). Here the same exact
+// integrals are a contraction of per-tet metric tensors with constant reference-element
+// tensors (ned2_tables.inc), which makes the kernel bound by its output stream.
+//
+// Per tet, with the 4 vertices put in ascending global-id order (the "canonical" order,
+// in which every edge/face function has its vertices in local order):
+//   J = [w1-w0, w2-w0, w3-w0],  C = rows (J1 x J2, J2 x J0, J0 x J1) = det(J) J^-1
+//   G_M = C eps C^T / |det J|          (mass:      f = J^-T f_hat)
+//   G_K = J^T mu^-1 J / |det J|        (stiffness: curl f = J curl f_hat / det J)
+//   X(a,b) = sum_pq Gs_pq TSYM[ab][pq] -/+ sum_pq Ga_pq TANTI[ab][pq]
+// then canonical function c goes to reference slot SLOT_OF[code][c], and entries with
+// exactly one odd-degree function (codes 66/128/129 vs 64) take the factor
+// s = sign(det of the stored-order Jacobian): the reference normalises barycentric
+// coefficients by |V| (mth/optimized.py:142-167, curlcurl.py:306-311).
+//
+// Work layout: one workgroup of 256 threads owns a tile of 64 consecutive assembly
+// indices. Phase 1: wave 0 computes the metric tensors of the 64 tets (FP64) into LDS.
+// Phase 2: thread j < 210 owns canonical pair j (a <= b) with its 12 (18) tensor
+// coefficients in registers, and for each sub-batch of NB tets writes its value into
+// both mirrored positions of an LDS copy of the 20x20 blocks. Phase 3: all 256 threads
+// stream the sub-batch (NB*400 contiguous COO entries) to HBM with 16-byte stores.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "em_internal.h"
+#include "ned2_tables.inc"
+
+namespace em {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 64;
+
+struct Cd {
+    double re, im;
+};
+__device__ __forceinline__ Cd cmul(Cd a, Cd b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ Cd cadd(Cd a, Cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cd csub(Cd a, Cd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cd cscale(Cd a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ Cd cdiv(Cd a, Cd b) {
+    double d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+// Tensor of one tet as a complex 3x3 (row-major).
+__device__ __forceinline__ void load_tensor(const MatDesc& m, long long t, Cd E[9]) {
+    const double2* p = m.p + t * m.ts;
+    if (m.kind == EM_MAT_SCALAR) {
+        double2 v = p[0];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) E[i] = {0.0, 0.0};
+        E[0] = E[4] = E[8] = {v.x, v.y};
+    } else if (m.kind == EM_MAT_DIAG) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) E[i] = {0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double2 v = p[d * m.cs];
+            E[4 * d] = {v.x, v.y};
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            double2 v = p[i * m.cs];
+            E[i] = {v.x, v.y};
+        }
+    }
+}
+
+// Inverse with the reference's semantics (mth/optimized.py:315-357): a tensor with
+// all off-diagonal entries exactly zero is inverted entry-wise, otherwise by cofactors.
+__device__ __forceinline__ void invert_tensor(Cd E[9]) {
+    bool diag = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        if (i != 0 && i != 4 && i != 8) diag = diag && (E[i].re == 0.0 && E[i].im == 0.0);
+    const Cd one = {1.0, 0.0};
+    if (diag) {
+        E[0] = cdiv(one, E[0]);
+        E[4] = cdiv(one, E[4]);
+        E[8] = cdiv(one, E[8]);
+        return;
+    }
+    Cd o[9];
+    o[0] = csub(cmul(E[4], E[8]), cmul(E[5], E[7]));
+    o[1] = csub(cmul(E[2], E[7]), cmul(E[1], E[8]));
+    o[2] = csub(cmul(E[1], E[5]), cmul(E[2], E[4]));
+    o[3] = csub(cmul(E[5], E[6]), cmul(E[3], E[8]));
+    o[4] = csub(cmul(E[0], E[8]), cmul(E[2], E[6]));
+    o[5] = csub(cmul(E[2], E[3]), cmul(E[0], E[5]));
+    o[6] = csub(cmul(E[3], E[7]), cmul(E[4], E[6]));
+    o[7] = csub(cmul(E[1], E[6]), cmul(E[0], E[7]));
+    o[8] = csub(cmul(E[0], E[4]), cmul(E[1], E[3]));
+    Cd det = cadd(cadd(cmul(E[0], o[0]), cmul(E[1], o[3])), cmul(E[2], o[6]));
+#pragma unroll
+    for (int i = 0; i < 9; ++i) E[i] = cdiv(o[i], det);
+}
+
+// G = A E A^T * scale for real 3x3 A (rows a0,a1,a2) and complex E (row-major).
+__device__ __forceinline__ void congruence(const double A[3][3], const Cd E[9], int kind,
+                                           double scale, Cd G[9]) {
+    if (kind == EM_MAT_SCALAR) {
+        // E = e I: G = e (A A^T) scale
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                double d = (A[p][0] * A[q][0] + A[p][1] * A[q][1] + A[p][2] * A[q][2]) * scale;
+                G[3 * p + q] = cscale(E[0], d);
+            }
+        return;
+    }
+    Cd AE[9];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Cd acc = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc = cadd(acc, cscale(E[3 * k + c], A[p][k]));
+            AE[3 * p + c] = acc;
+        }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            Cd acc = {0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc = cadd(acc, cscale(AE[3 * p + c], A[q][c]));
+            G[3 * p + q] = cscale(acc, scale);
+        }
+}
+
+// LDS record of one tet: symmetric (6) and antisymmetric (3) parts of G_K and G_M,
+// the rank code and the orientation sign.
+template <typename R>
+struct TileRec {
+    R gk[kTile][6][2];
+    R gm[kTile][6][2];
+    R ak[kTile][3][2];
+    R am[kTile][3][2];
+    int code[kTile];
+    R sgn[kTile];
+};
+
+template <typename R>
+__device__ __forceinline__ void split_store(const Cd G[9], R (*sym)[2], R (*anti)[2]) {
+    const Cd s01 = cscale(cadd(G[1], G[3]), 0.5), s02 = cscale(cadd(G[2], G[6]), 0.5),
+             s12 = cscale(cadd(G[5], G[7]), 0.5);
+    const Cd a01 = cscale(csub(G[1], G[3]), 0.5), a02 = cscale(csub(G[2], G[6]), 0.5),
+             a12 = cscale(csub(G[5], G[7]), 0.5);
+    const Cd sv[6] = {G[0], G[4], G[8], s01, s02, s12};
+    const Cd av[3] = {a01, a02, a12};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        sym[c][0] = (R)sv[c].re;
+        sym[c][1] = (R)sv[c].im;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        anti[c][0] = (R)av[c].re;
+        anti[c][1] = (R)av[c].im;
+    }
+}
+
+// Phase 1 for one tet (FP64 throughout).
+template <typename R>
+__device__ __forceinline__ void tet_geometry(const MeshDev& m, const MatDesc& eps,
+                                             const MatDesc& mu, long long ia, int lt,
+                                             TileRec<R>& rec) {
+    const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
+    const int4 nd = m.tets[t];
+    const int n[4] = {nd.x, nd.y, nd.z, nd.w};
+    int r[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
+    double4 X[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) X[a] = m.xyz[n[a]];
+    double W[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double4 v = (r[0] == k) ? X[0] : (r[1] == k) ? X[1] : (r[2] == k) ? X[2] : X[3];
+        W[k][0] = v.x;
+        W[k][1] = v.y;
+        W[k][2] = v.z;
+    }
+    double Jc[3][3];  // Jc[q] = column q = w_{q+1} - w_0
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Jc[q][c] = W[q + 1][c] - W[0][c];
+    double C[3][3];  // rows: J1 x J2, J2 x J0, J0 x J1
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const double* u = Jc[(p + 1) % 3];
+        const double* v = Jc[(p + 2) % 3];
+        C[p][0] = u[1] * v[2] - u[2] * v[1];
+        C[p][1] = u[2] * v[0] - u[0] * v[2];
+        C[p][2] = u[0] * v[1] - u[1] * v[0];
+    }
+    const double det = Jc[0][0] * C[0][0] + Jc[0][1] * C[0][1] + Jc[0][2] * C[0][2];
+    const double adet = fabs(det);
+    const int code = r[0] * 16 + r[1] * 4 + r[2];
+    const int inv = (r[0] > r[1]) + (r[0] > r[2]) + (r[0] > r[3]) + (r[1] > r[2]) +
+                    (r[1] > r[3]) + (r[2] > r[3]);
+    const double s = ((det < 0.0) != (inv & 1)) ? -1.0 : 1.0;
+
+    Cd E[9], G[9];
+    load_tensor(eps, t, E);
+    congruence(C, E, eps.kind, 1.0 / adet, G);
+    split_store<R>(G, rec.gm[lt], rec.am[lt]);
+
+    load_tensor(mu, t, E);
+    if (mu.invert) invert_tensor(E);
+    double JT[3][3];  // rows of J^T = columns of J
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) JT[q][c] = Jc[q][c];
+    congruence(JT, E, mu.kind, 1.0 / adet, G);
+    split_store<R>(G, rec.gk[lt], rec.ak[lt]);
+    rec.code[lt] = code;
+    rec.sgn[lt] = (R)s;
+}
+
+template <typename R>
+struct alignas(2 * sizeof(R)) C2 {
+    R re, im;
+};
+
+template <typename R>
+struct OutVec;
+template <>
+struct OutVec<double> {
+    using T = double2;
+};
+template <>
+struct OutVec<float> {
+    using T = float2;
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Copy `bytes` (multiple of 16) from LDS to global with 16-byte streaming stores.
+__device__ __forceinline__ void stream_out(const void* lds, void* gbl, int bytes) {
+    const u32x4* s = reinterpret_cast<const u32x4*>(lds);
+    u32x4* d = reinterpret_cast<u32x4*>(gbl);
+    const int n = bytes >> 4;
+    for (int i = threadIdx.x; i < n; i += kBlock) __builtin_nontemporal_store(s[i], d + i);
+}
+
+template <typename R, int MODE, bool ANTI, int NB>
+struct Smem {
+    static constexpr int kStageMats = (MODE == MODE_A) ? 1 : 2;
+    TileRec<R> rec;
+    alignas(16) C2<R> stage[2][kStageMats][NB * 400];
+    unsigned char slot[64][20];
+};
+
+template <typename R, int MODE, bool ANTI, int NB>
+__global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
+    MeshDev m, MatDesc eps, MatDesc mu, const double* __restrict__ k0sq, int nf,
+    double k0sq0, void* __restrict__ out0, void* __restrict__ out1, long long freq_stride) {
+    using O = typename OutVec<R>::T;
+    __shared__ Smem<R, MODE, ANTI, NB> sm;
+    constexpr int NS = Smem<R, MODE, ANTI, NB>::kStageMats;
+
+    for (int i = threadIdx.x; i < 64 * 20; i += kBlock) sm.slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
+
+    const long long tile0 = (long long)blockIdx.x * kTile;
+    const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
+    if (threadIdx.x < ntile) tet_geometry<R>(m, eps, mu, tile0 + threadIdx.x, threadIdx.x, sm.rec);
+
+    // Pair coefficients of this thread (registers).
+    const int j = threadIdx.x;
+    const bool active = j < NED2_NPAIR;
+    const int pj = active ? j : 0;
+    const int a = PAIR_A[pj], b = PAIR_B[pj];
+    R tk[6], tm[6], qk[3], qm[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        tk[c] = (R)TSYM_K[pj][c];
+        tm[c] = (R)TSYM_M[pj][c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        qk[c] = ANTI ? (R)TANTI_K[pj][c] : (R)0;
+        qm[c] = ANTI ? (R)TANTI_M[pj][c] : (R)0;
+    }
+    const bool aodd = a >= 6, bodd = b >= 6;
+    const R kq = (R)k0sq0;
+    __syncthreads();
+
+    const int nsub = (ntile + NB - 1) / NB;
+    for (int sb = 0; sb < nsub; ++sb) {
+        const int buf = sb & 1;
+        const int lt0 = sb * NB;
+        const int nvalid = min(NB, ntile - lt0);
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (u < nvalid) {
+                    const int lt = lt0 + u;
+                    const int code = sm.rec.code[lt];
+                    const int ra = sm.slot[code][a], rb = sm.slot[code][b];
+                    R kr = 0, ki = 0, mr = 0, mi = 0;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        kr += tk[c] * sm.rec.gk[lt][c][0];
+                        ki += tk[c] * sm.rec.gk[lt][c][1];
+                        mr += tm[c] * sm.rec.gm[lt][c][0];
+                        mi += tm[c] * sm.rec.gm[lt][c][1];
+                    }
+                    if (ANTI) {
+                        R xr = 0, xi = 0, yr = 0, yi = 0;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            xr += qk[c] * sm.rec.ak[lt][c][0];
+                            xi += qk[c] * sm.rec.ak[lt][c][1];
+                            yr += qm[c] * sm.rec.am[lt][c][0];
+                            yi += qm[c] * sm.rec.am[lt][c][1];
+                        }
+                        const R sa = (ra < rb) ? (R)-1 : (R)1;
+                        kr += sa * xr;
+                        ki += sa * xi;
+                        mr += sa * yr;
+                        mi += sa * yi;
+                    }
+                    const R s = sm.rec.sgn[lt];
+                    const R sg = (aodd ? s : (R)1) * (bodd ? s : (R)1);
+                    const int e1 = ra * 20 + rb, e2 = rb * 20 + ra;
+                    if (MODE == MODE_A) {
+                        C2<R> v = {sg * (kr - kq * mr), sg * (ki - kq * mi)};
+                        sm.stage[buf][0][u * 400 + e1] = v;
+                        sm.stage[buf][0][u * 400 + e2] = v;
+                    } else {
+                        C2<R> vk = {sg * kr, sg * ki}, vm = {sg * mr, sg * mi};
+                        sm.stage[buf][0][u * 400 + e1] = vk;
+                        sm.stage[buf][0][u * 400 + e2] = vk;
+                        sm.stage[buf][1][u * 400 + e1] = vm;
+                        sm.stage[buf][1][u * 400 + e2] = vm;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const long long ia0 = tile0 + lt0;
+        const int nent = nvalid * 400;
+        if (MODE == MODE_A) {
+            O* dst = reinterpret_cast<O*>(out0) + ia0 * 400;
+            stream_out(&sm.stage[buf][0][0], dst, nent * (int)sizeof(O));
+        } else if (MODE == MODE_KM) {
+            stream_out(&sm.stage[buf][0][0], reinterpret_cast<O*>(out0) + ia0 * 400,
+                       nent * (int)sizeof(O));
+            stream_out(&sm.stage[buf][1][0], reinterpret_cast<O*>(out1) + ia0 * 400,
+                       nent * (int)sizeof(O));
+        } else {
+            // A_f = K - k0_f^2 M for every wavenumber, 16 bytes per store.
+            constexpr int per = 16 / (int)sizeof(O);  // complex entries per store
+            const int nvec = nent / per;
+            for (int f = 0; f < nf; ++f) {
+                const R kf = (R)k0sq[f];
+                u32x4* dst = reinterpret_cast<u32x4*>(reinterpret_cast<O*>(out0) +
+                                                      (long long)f * freq_stride + ia0 * 400);
+                for (int i = threadIdx.x; i < nvec; i += kBlock) {
+                    union {
+                        O o[per];
+                        u32x4 v;
+                    } pk;
+#pragma unroll
+                    for (int q = 0; q < per; ++q) {
+                        const C2<R> K = sm.stage[buf][0][i * per + q];
+                        const C2<R> M = sm.stage[buf][NS - 1][i * per + q];
+                        pk.o[q].x = K.re - kf * M.re;
+                        pk.o[q].y = K.im - kf * M.im;
+                    }
+                    __builtin_nontemporal_store(pk.v, dst + i);
+                }
+            }
+        }
+        // The next sub-batch writes the other buffer; the barrier above already ordered
+        // every read of that buffer (two sub-batches ago) before this point.
+    }
+}
+
+template <typename R, int MODE, bool ANTI, int NB>
+static hipError_t launch_t(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
+                           const double* k0sq_dev, int nf, double k0sq0, void* out0, void* out1,
+                           long long freq_stride, hipStream_t stream) {
+    const long long ntiles = (m.n_assy + kTile - 1) / kTile;
+    if (ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, ANTI, NB>), dim3((unsigned)ntiles),
+                       dim3(kBlock), 0, stream, m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1,
+                       freq_stride);
+    return hipGetLastError();
+}
+
+template <typename R, int MODE>
+static hipError_t launch_anti(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int anti,
+                              const double* k0sq_dev, int nf, double k0sq0, void* out0,
+                              void* out1, long long freq_stride, int nb, hipStream_t stream) {
+    (void)nb;
+    if (anti)
+        return launch_t<R, MODE, true, 2>(m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1,
+                                          freq_stride, stream);
+    return launch_t<R, MODE, false, 2>(m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1, freq_stride,
+                                       stream);
+}
+
+template <typename R>
+static hipError_t launch_mode(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
+                              int anti, const double* k0sq_dev, const double* k0sq_host, int nf,
+                              void* out0, void* out1, long long freq_stride, int nb,
+                              hipStream_t stream) {
+    if (mode == MODE_A)
+        return launch_anti<R, MODE_A>(m, eps, mu, anti, k0sq_dev, nf, k0sq_host[0], out0, out1,
+                                      freq_stride, nb, stream);
+    if (mode == MODE_KM)
+        return launch_anti<R, MODE_KM>(m, eps, mu, anti, k0sq_dev, nf, 0.0, out0, out1,
+                                       freq_stride, nb, stream);
+    return launch_anti<R, MODE_AF>(m, eps, mu, anti, k0sq_dev, nf, 0.0, out0, out1, freq_stride,
+                                   nb, stream);
+}
+
+hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
+                           int precision, int anti, const double* k0sq_dev,
+                           const double* k0sq_host, int nf, void* out0, void* out1,
+                           long long freq_stride, int nb, hipStream_t stream) {
+    if (precision == EM_PREC_FP64)
+        return launch_mode<double>(m, eps, mu, mode, anti, k0sq_dev, k0sq_host, nf, out0, out1,
+                                   freq_stride, nb, stream);
+    return launch_mode<float>(m, eps, mu, mode, anti, k0sq_dev, k0sq_host, nf, out0, out1,
+                              freq_stride, nb, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Static COO pattern: rows/cols of every entry (curlcurl.py:449-453).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ned2_rowcol_kernel(const int* __restrict__ t2f,
+                                                          const int* __restrict__ tetids,
+                                                          long long n_assy,
+                                                          long long* __restrict__ rows,
+                                                          long long* __restrict__ cols) {
+    const long long n = n_assy * 400;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long ia = e / 400;
+        const int r = (int)(e - ia * 400);
+        const long long t = tetids ? (long long)tetids[ia] : ia;
+        const int* d = t2f + t * 20;
+        rows[e] = d[r / 20];
+        cols[e] = d[r % 20];
+    }
+}
+
+hipError_t launch_rowcol(const MeshDev& m, long long* rows, long long* cols, hipStream_t s) {
+    const long long n = m.n_assy * 400;
+    if (n == 0) return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(ned2_rowcol_kernel, dim3((unsigned)blocks), dim3(256), 0, s, m.t2f,
+                       m.tetids, m.n_assy, rows, cols);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Plan validation: edge/face tables equal the sorted tuples of each tet's slots
+// (mesh3d.py:707-774), and no tet has zero volume.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sort2(long long& a, long long& b) {
+    if (b < a) {
+        long long t = a;
+        a = b;
+        b = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void check_tables_kernel(
+    const int4* __restrict__ tets, long long T, const long long* __restrict__ edges, long long E,
+    const long long* __restrict__ tris, long long F, const long long* __restrict__ t2e,
+    const long long* __restrict__ t2t, unsigned long long* bad) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int4 nd = tets[t];
+    const long long n[4] = {nd.x, nd.y, nd.z, nd.w};
+    const int es[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {3, 1}, {2, 3}};
+    const int fs[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 2, 3}};
+    unsigned long long nbad = 0;
+    for (int s = 0; s < 6; ++s) {
+        long long a = n[es[s][0]], b = n[es[s][1]];
+        sort2(a, b);
+        const long long id = t2e[s * T + t];
+        if (id < 0 || id >= E || edges[id] != a || edges[E + id] != b) ++nbad;
+    }
+    for (int s = 0; s < 4; ++s) {
+        long long a = n[fs[s][0]], b = n[fs[s][1]], c = n[fs[s][2]];
+        sort2(a, b);
+        sort2(b, c);
+        sort2(a, b);
+        const long long id = t2t[s * T + t];
+        if (id < 0 || id >= F || tris[id] != a || tris[F + id] != b || tris[2 * F + id] != c)
+            ++nbad;
+    }
+    if (nbad) atomicAdd(bad, nbad);
+}
+
+hipError_t launch_check_tables(const int4* tets, long long T, const long long* edges, long long E,
+                               const long long* tris, long long F, const long long* t2e,
+                               const long long* t2t, unsigned long long* bad, hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(check_tables_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s,
+                       tets, T, edges, E, tris, F, t2e, t2t, bad);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void check_volumes_kernel(const double4* __restrict__ xyz,
+                                                            const int4* __restrict__ tets,
+                                                            long long T,
+                                                            unsigned long long* bad) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int4 nd = tets[t];
+    const double4 p0 = xyz[nd.x], p1 = xyz[nd.y], p2 = xyz[nd.z], p3 = xyz[nd.w];
+    const double ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+    const double bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+    const double cx = p3.x - p0.x, cy = p3.y - p0.y, cz = p3.z - p0.z;
+    const double det = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
+    if (!(det != 0.0) || !isfinite(det)) atomicAdd(bad, 1ull);
+}
+
+hipError_t launch_check_volumes(const double4* xyz, const int4* tets, long long T,
+                                unsigned long long* bad, hipStream_t s) {
+    if (T == 0) return hipSuccess;
+    hipLaunchKernelGGL(check_volumes_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s,
+                       xyz, tets, T, bad);
+    return hipGetLastError();
+}
+
+}  // namespace em

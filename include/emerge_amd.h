@@ -1,0 +1,126 @@
+/*
+ * emerge_amd -- MI355X (gfx950) Nedelec-2 tetrahedral curl-curl system assembly.
+ *
+ * C-ABI of libemerge_amd.so. Plain pointers and sizes; no C++ types cross it. Every
+ * function returns an int status (EM_OK = 0, negative on error); em_last_error()
+ * returns a thread-local message for the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths under src/emerge/_emerge/ of EMerge 2.8.0):
+ *   physics/microwave/assembly/curlcurl.py:387-456  _matrix_builder(nodes, tets, tris,
+ *       edges, tet_to_field, tet_to_edge, tet_to_tri, ur, er, tetids, dofcodes)
+ *       -> (dataE, dataB, rows, cols)
+ *   physics/microwave/assembly/curlcurl.py:222-270  tet_mass_stiffness_matrices
+ *   mth/csc_cast.py:51-53, 65-160                   CSCMapping.from_rowcol /
+ *                                                   precompute_csc_pattern
+ *   mth/csc_cast.py:55-63, 162-177                  CSCMapping.to_csc / scatter_to_csc_c16
+ *   physics/microwave/assembly/assembler.py:494      to_csc(Evec - Bvec * K0**2)
+ *
+ * Array layouts on input follow the reference's numpy arrays (C order, int64 ids):
+ *   nodes (3, Nn) f8, tets (4, T), tris (3, F), edges (2, E), tet_to_field (20, T),
+ *   tet_to_edge (6, T), tet_to_tri (4, T), tetids (Ta,), dofcodes (20,).
+ * Host pointers are used for the mesh tables given to em_ned2_plan_create (copied to
+ * the device once). Material tensors, k0 values and every output are DEVICE pointers;
+ * launches are asynchronous on the given HIP stream (hipStream_t passed as void*,
+ * NULL = default stream).
+ *
+ * COO output layout (identical to curlcurl.py:449-455): the 20x20 block of assembly
+ * index ia (tet tetids[ia]) occupies entries [400*ia, 400*ia + 400), row-major
+ * (entry 20*ii + jj has row tet_to_field[ii], col tet_to_field[jj]).
+ */
+#ifndef EMERGE_AMD_H
+#define EMERGE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define EM_OK 0
+#define EM_EINVAL (-1)        /* bad shape, dtype, size or argument */
+#define EM_EUNSUPPORTED (-2)  /* DOF set / DOF code not supported (ccbf.py:1206-1207) */
+#define EM_EHIP (-3)          /* HIP runtime error */
+#define EM_ENOMEM (-4)        /* device allocation failed */
+#define EM_EDEGENERATE (-5)   /* zero-volume tet (reference divides by V = 0) */
+#define EM_ETABLES (-6)       /* edge/face tables inconsistent with the tets */
+
+/* ---- material tensor kinds (complex128, interleaved re/im) ---- */
+#define EM_MAT_SCALAR 0  /* one value per tet: tensor = v * I */
+#define EM_MAT_DIAG 1    /* 3 values per tet: diag(v0, v1, v2) */
+#define EM_MAT_FULL 2    /* 9 values per tet, row-major (p, q) */
+
+/* flags of em_material_t */
+#define EM_MAT_IS_INVERSE 1 /* (mu only) data already holds mu_r^-1: skip matinv */
+
+/*
+ * A per-tet tensor field. value(t, c) = data[t * tet_stride + c * comp_stride]
+ * (complex128 elements). The reference's (3, 3, T) arrays are EM_MAT_FULL with
+ * tet_stride = 1, comp_stride = T. tet_stride = 0 gives one tensor for all tets.
+ */
+typedef struct {
+    const void* data;      /* device pointer, complex128 */
+    int32_t kind;          /* EM_MAT_* */
+    int32_t flags;         /* EM_MAT_IS_INVERSE or 0 */
+    int64_t tet_stride;    /* in complex elements, indexed by TET id (not assembly idx) */
+    int64_t comp_stride;   /* in complex elements */
+} em_material_t;
+
+/* ---- precision / output type ---- */
+#define EM_PREC_FP64 0   /* FP64 geometry and contraction, complex128 output */
+#define EM_PREC_MIXED 1  /* FP64 geometry, FP32 contraction, complex64 output */
+
+typedef struct em_ned2_plan_s* em_ned2_plan_t;
+typedef struct em_csc_map_s* em_csc_map_t;
+
+const char* em_last_error(void);
+const char* em_version(void);
+
+/*
+ * Create an assembly plan: copies the mesh tables to device `device`, validates
+ * them (ids in range, edge/face tables equal to the sorted-tuple tables of the tets as
+ * mesh3d.py:707-774 builds them, no zero-volume tet) and keeps them resident.
+ * Replaces the per-call table arguments of _matrix_builder (curlcurl.py:387-417).
+ * tetids may be NULL (all tets, in order); otherwise it lists the Ta assembled tets
+ * (the conductor-free compaction of curlcurl.py:248-250). dofcodes must be the
+ * SECOND_MIXED_VOLAKIS codes [64]*6 + [66]*6 + [128]*4 + [129]*4.
+ */
+int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes,
+                        int64_t n_nodes, const int64_t* tets, int64_t n_tets,
+                        const int64_t* tris, int64_t n_tris, const int64_t* edges,
+                        int64_t n_edges, const int64_t* tet_to_field,
+                        const int64_t* tet_to_edge, const int64_t* tet_to_tri,
+                        int64_t n_field, const int64_t* tetids, int64_t n_assy,
+                        const int64_t* dofcodes, int32_t ndof);
+int em_ned2_plan_destroy(em_ned2_plan_t plan);
+/* n_assy (Ta), n_field (N), entries per COO array (400 * Ta) */
+int em_ned2_plan_sizes(em_ned2_plan_t plan, int64_t* n_assy, int64_t* n_field,
+                       int64_t* nnz_coo);
+
+/*
+ * Element stiffness and mass blocks, reference-style (curlcurl.py:387-456):
+ * dataE = K (with mu_r^-1), dataB = M (with eps_r), complex128 [400 * Ta] each
+ * (precision EM_PREC_FP64) or complex64 (EM_PREC_MIXED).
+ */
+int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er,
+                        const em_material_t* ur, void* dataE, void* dataB,
+                        int32_t precision, void* stream);
+
+/*
+ * Fused A_f = K - k0[f]^2 M for n_k0 wavenumbers (assembler.py:485-494 combine step,
+ * batched over a frequency sweep). out holds n_k0 COO value arrays, array f starting
+ * at element f * freq_stride (freq_stride >= 400 * Ta). k0 is a HOST array.
+ * complex128 for EM_PREC_FP64, complex64 for EM_PREC_MIXED.
+ */
+int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er,
+                       const em_material_t* ur, const double* k0, int32_t n_k0, void* out,
+                       int64_t freq_stride, int32_t precision, void* stream);
+
+/* The static COO pattern: rows/cols int64 [400 * Ta] (curlcurl.py:449-453). */
+int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMERGE_AMD_H */

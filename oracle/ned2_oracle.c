@@ -1,0 +1,415 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * A CPU restatement, in C with OpenMP, of the reference's Nedelec-2 tetrahedral
+ * curl-curl assembly and its COO->CSC map. Only the test suite, the smoke check and
+ * the CPU-baseline leg of the benchmark load this library, as the checker; the
+ * product path never calls it.
+ *
+ * Followed step by step (paths relative to src/emerge/_emerge/ in the reference):
+ *   or_tet_coefficients   mth/optimized.py:127-214   cofactors of the 4x4 vertex
+ *                         matrix and V = |det|/6 (the |V| quirk: coeff = cof/(6V)
+ *                         is sign(det) times the true barycentric coefficients)
+ *   or_local_map          mth/optimized.py:262-296, physics/.../curlcurl.py:72-85
+ *   or_matinv             mth/optimized.py:315-357 (diagonal fast path kept)
+ *   basis / curl          compiled/ccbf.py 3D functions for codes 64 (edge fn 0),
+ *                         66 (edge fn 2), 128 (face fn 0), 129 (face fn 1):
+ *                         _ne0_3d :536, _ne2_3d :568, _nf0_3d :789, _nf1_3d :807,
+ *                         _curl_ne0_3d :643, _curl_ne2_3d :658, _curl_nf0_3d :927,
+ *                         _curl_nf1_3d :948; written here from their definitions
+ *                         (elements/dofsets.py:66-83), curls derived by hand
+ *   or_parse_dofcode      compiled/ccbf.py:40-66
+ *   or_tet_stiff_mass     physics/microwave/assembly/curlcurl.py:285-380 (11-point
+ *                         Keast rule of :277-283, complex arithmetic, i<=j then mirror)
+ *   ned2_oracle_build     curlcurl.py:387-456 (_matrix_builder; COO block per tet,
+ *                         row-major, p = iassy*ndof^2)
+ *   ned2_oracle_csc_*     mth/csc_cast.py:65-177 (precompute_csc_pattern, scatter)
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double _Complex cplx;
+
+/* 11-point Keast degree-4 rule: weights then barycentric points
+ * (curlcurl.py:277-283, identical to keast_rules.py KEAST4). */
+static const double DPTS[5][11] = {
+    {-0.07893333333333329982, 0.04573333333333329948, 0.04573333333333329948,
+     0.04573333333333329948, 0.04573333333333329948, 0.14933333333333329018,
+     0.14933333333333329018, 0.14933333333333329018, 0.14933333333333329018,
+     0.14933333333333329018, 0.14933333333333329018},
+    {0.25000000000000000000, 0.78571428571428569843, 0.07142857142857139685,
+     0.07142857142857139685, 0.07142857142857139685, 0.10059642383320080428,
+     0.39940357616679922348, 0.39940357616679922348, 0.39940357616679922348,
+     0.10059642383320080428, 0.10059642383320080428},
+    {0.25000000000000000000, 0.07142857142857139685, 0.07142857142857139685,
+     0.07142857142857139685, 0.78571428571428569843, 0.39940357616679922348,
+     0.10059642383320080428, 0.39940357616679922348, 0.10059642383320080428,
+     0.39940357616679922348, 0.10059642383320080428},
+    {0.25000000000000000000, 0.07142857142857139685, 0.07142857142857139685,
+     0.78571428571428569843, 0.07142857142857139685, 0.39940357616679922348,
+     0.39940357616679922348, 0.10059642383320080428, 0.10059642383320080428,
+     0.10059642383320080428, 0.39940357616679922348},
+    {0.25000000000000000000, 0.07142857142857150787, 0.78571428571428580945,
+     0.07142857142857150787, 0.07142857142857150787, 0.10059642383320077652,
+     0.10059642383320077652, 0.10059642383320074877, 0.39940357616679922348,
+     0.39940357616679922348, 0.39940357616679922348},
+};
+#define NQ 11
+
+enum { OR_OK = 0, OR_EBADCODE = -2, OR_EDEGEN = -5 };
+
+static double det3(double a0, double a1, double a2, double b0, double b1, double b2,
+                   double c0, double c1, double c2) {
+    return a0 * (b1 * c2 - b2 * c1) - a1 * (b0 * c2 - b2 * c0) + a2 * (b0 * c1 - b1 * c0);
+}
+
+/* Cofactors C[n][k] of P = [[1,x_n,y_n,z_n]] (row n, column k) and |det P|/6.
+ * a_n = C[n][0], b_n = C[n][1], c_n = C[n][2], d_n = C[n][3]. */
+static void or_tet_coefficients(const double x[4], const double y[4], const double z[4],
+                                double a[4], double b[4], double c[4], double d[4],
+                                double *V) {
+    for (int n = 0; n < 4; ++n) {
+        int r[3], m = 0;
+        for (int k = 0; k < 4; ++k)
+            if (k != n) r[m++] = k;
+        double sg = (n % 2 == 0) ? 1.0 : -1.0;
+        a[n] = sg * det3(x[r[0]], y[r[0]], z[r[0]], x[r[1]], y[r[1]], z[r[1]],
+                         x[r[2]], y[r[2]], z[r[2]]);
+        b[n] = -sg * det3(1.0, y[r[0]], z[r[0]], 1.0, y[r[1]], z[r[1]], 1.0, y[r[2]], z[r[2]]);
+        c[n] = sg * det3(1.0, x[r[0]], z[r[0]], 1.0, x[r[1]], z[r[1]], 1.0, x[r[2]], z[r[2]]);
+        d[n] = -sg * det3(1.0, x[r[0]], y[r[0]], 1.0, x[r[1]], y[r[1]], 1.0, x[r[2]], y[r[2]]);
+    }
+    /* det P = sum_n C[n][0] (expansion along the column of ones) */
+    *V = fabs(a[0] + a[1] + a[2] + a[3]) / 6.0;
+}
+
+/* Local vertex index of every global id of a small table column (optimized.py:262). */
+static void or_local_map(const int64_t vid[4], const int64_t *gids, int nrow, int64_t *out) {
+    for (int j = 0; j < nrow; ++j) {
+        out[j] = 0;
+        for (int k = 0; k < 4; ++k)
+            if (vid[k] == gids[j]) { out[j] = k; break; }
+    }
+}
+
+static void or_matinv(const cplx M[9], cplx out[9]) {
+    if (M[1] == 0 && M[2] == 0 && M[3] == 0 && M[5] == 0 && M[6] == 0 && M[7] == 0) {
+        for (int i = 0; i < 9; ++i) out[i] = 0;
+        out[0] = 1.0 / M[0];
+        out[4] = 1.0 / M[4];
+        out[8] = 1.0 / M[8];
+        return;
+    }
+    cplx det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+               M[2] * (M[3] * M[7] - M[4] * M[6]);
+    out[0] = (M[4] * M[8] - M[5] * M[7]) / det;
+    out[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+    out[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+    out[3] = (M[5] * M[6] - M[3] * M[8]) / det;
+    out[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+    out[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+    out[6] = (M[3] * M[7] - M[4] * M[6]) / det;
+    out[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+    out[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+}
+
+static inline void cross3(const double u[3], const double v[3], double w[3]) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+/* Basis function and curl at one point. l[4] barycentric values, g[4][3] gradients
+ * (both carry the |V| sign of the coefficients). */
+static int or_basis(int code, int i, int j, int k, const double l[4], const double g[4][3],
+                    double f[3], double cf[3]) {
+    double gij[3], w[3], t[3];
+    switch (code) {
+    case 64: /* L_i grad L_j - L_j grad L_i ; curl = 2 grad L_i x grad L_j */
+        cross3(g[i], g[j], gij);
+        for (int q = 0; q < 3; ++q) {
+            f[q] = l[i] * g[j][q] - l[j] * g[i][q];
+            cf[q] = 2.0 * gij[q];
+        }
+        return OR_OK;
+    case 66: /* (L_i - L_j)(L_i grad L_j - L_j grad L_i) ; curl = 3 (L_i - L_j) gi x gj */
+        cross3(g[i], g[j], gij);
+        for (int q = 0; q < 3; ++q) {
+            f[q] = (l[i] - l[j]) * (l[i] * g[j][q] - l[j] * g[i][q]);
+            cf[q] = 3.0 * (l[i] - l[j]) * gij[q];
+        }
+        return OR_OK;
+    case 128: /* L_i (L_j grad L_k - L_k grad L_j) */
+        for (int q = 0; q < 3; ++q) w[q] = l[j] * g[k][q] - l[k] * g[j][q];
+        cross3(g[i], w, t);
+        cross3(g[j], g[k], gij);
+        for (int q = 0; q < 3; ++q) {
+            f[q] = l[i] * w[q];
+            cf[q] = t[q] + 2.0 * l[i] * gij[q];
+        }
+        return OR_OK;
+    case 129: /* L_j (L_i grad L_k - L_k grad L_i) */
+        for (int q = 0; q < 3; ++q) w[q] = l[i] * g[k][q] - l[k] * g[i][q];
+        cross3(g[j], w, t);
+        cross3(g[i], g[k], gij);
+        for (int q = 0; q < 3; ++q) {
+            f[q] = l[j] * w[q];
+            cf[q] = t[q] + 2.0 * l[j] * gij[q];
+        }
+        return OR_OK;
+    default:
+        return OR_EBADCODE;
+    }
+}
+
+/* compiled/ccbf.py:40-66: type (0 edge, 1 face) and occurrence index per code. */
+static void or_parse_dofcode(const int64_t *codes, int ndof, int64_t *typ, int64_t *idx) {
+    int64_t ne[64] = {0}, nf[64] = {0};
+    for (int i = 0; i < ndof; ++i) {
+        int64_t c = codes[i] & 63;
+        if ((codes[i] & 192) == 64) {
+            typ[i] = 0;
+            idx[i] = ne[c]++;
+        } else {
+            typ[i] = 1;
+            idx[i] = nf[c]++;
+        }
+    }
+}
+
+#define MAXDOF 20
+
+/* curlcurl.py:294-380 for one tet. vx[3][4] vertex coords in stored order. */
+static int or_tet_stiff_mass(const double vx[3][4], const int64_t lem[2][6],
+                             const int64_t ltm[3][4], const cplx Ms[9], const cplx Mm[9],
+                             const int64_t *codes, const int64_t *typ, const int64_t *idx,
+                             int ndof, cplx *Kout, cplx *Mout) {
+    double a[4], b[4], c[4], d[4], V;
+    or_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
+    if (!(V > 0.0)) return OR_EDEGEN;
+    double co[4][4]; /* co[0]=a/(6V), co[1]=b/(6V) ... (curlcurl.py:307-311) */
+    for (int n = 0; n < 4; ++n) {
+        co[0][n] = a[n] / (6.0 * V);
+        co[1][n] = b[n] / (6.0 * V);
+        co[2][n] = c[n] / (6.0 * V);
+        co[3][n] = d[n] / (6.0 * V);
+    }
+    static __thread cplx F[MAXDOF][3][NQ], CF[MAXDOF][3][NQ], MsCF[MAXDOF][3][NQ],
+        MmF[MAXDOF][3][NQ];
+    double g[4][3];
+    for (int n = 0; n < 4; ++n) {
+        g[n][0] = co[1][n];
+        g[n][1] = co[2][n];
+        g[n][2] = co[3][n];
+    }
+    for (int q = 0; q < NQ; ++q) {
+        double px = vx[0][0] * DPTS[1][q] + vx[0][1] * DPTS[2][q] + vx[0][2] * DPTS[3][q] +
+                    vx[0][3] * DPTS[4][q];
+        double py = vx[1][0] * DPTS[1][q] + vx[1][1] * DPTS[2][q] + vx[1][2] * DPTS[3][q] +
+                    vx[1][3] * DPTS[4][q];
+        double pz = vx[2][0] * DPTS[1][q] + vx[2][1] * DPTS[2][q] + vx[2][2] * DPTS[3][q] +
+                    vx[2][3] * DPTS[4][q];
+        double l[4];
+        for (int n = 0; n < 4; ++n) l[n] = co[0][n] + co[1][n] * px + co[2][n] * py + co[3][n] * pz;
+        for (int i = 0; i < ndof; ++i) {
+            int i1, j1, k1;
+            if (typ[i] == 0) {
+                i1 = (int)lem[0][idx[i]];
+                j1 = (int)lem[1][idx[i]];
+                k1 = 0;
+            } else {
+                i1 = (int)ltm[0][idx[i]];
+                j1 = (int)ltm[1][idx[i]];
+                k1 = (int)ltm[2][idx[i]];
+            }
+            double f[3], cf[3];
+            int st = or_basis((int)codes[i], i1, j1, k1, l, g, f, cf);
+            if (st) return st;
+            for (int r = 0; r < 3; ++r) {
+                F[i][r][q] = f[r];
+                CF[i][r][q] = cf[r];
+            }
+        }
+    }
+    for (int i = 0; i < ndof; ++i) {
+        for (int q = 0; q < NQ; ++q) {
+            for (int r = 0; r < 3; ++r) {
+                MsCF[i][r][q] = Ms[3 * r + 0] * CF[i][0][q] + Ms[3 * r + 1] * CF[i][1][q] +
+                                Ms[3 * r + 2] * CF[i][2][q];
+                MmF[i][r][q] = Mm[3 * r + 0] * F[i][0][q] + Mm[3 * r + 1] * F[i][1][q] +
+                               Mm[3 * r + 2] * F[i][2][q];
+            }
+            for (int r = 0; r < 3; ++r) {
+                MsCF[i][r][q] *= DPTS[0][q];
+                MmF[i][r][q] *= DPTS[0][q];
+            }
+        }
+    }
+    for (int i = 0; i < ndof; ++i) {
+        for (int j = i; j < ndof; ++j) {
+            cplx aS = 0, aM = 0;
+            for (int q = 0; q < NQ; ++q) {
+                aS += MsCF[i][0][q] * CF[j][0][q] + MsCF[i][1][q] * CF[j][1][q] +
+                      MsCF[i][2][q] * CF[j][2][q];
+                aM += MmF[i][0][q] * F[j][0][q] + MmF[i][1][q] * F[j][1][q] +
+                      MmF[i][2][q] * F[j][2][q];
+            }
+            Kout[i * ndof + j] = aS * V;
+            Mout[i * ndof + j] = aM * V;
+            Kout[j * ndof + i] = aS * V;
+            Mout[j * ndof + i] = aM * V;
+        }
+    }
+    return OR_OK;
+}
+
+/* _matrix_builder (curlcurl.py:387-456). Layouts follow the reference arrays:
+ * nodes[c*Nn+n], tets[v*T+t], tris[v*F+f], edges[v*E+e], tet_to_field[d*T+t],
+ * tet_to_edge[s*T+t], tet_to_tri[s*T+t], ur/er interleaved complex at [(p*3+q)*T+t].
+ * Outputs dataE/dataB (interleaved complex) and rows/cols have Ta*ndof^2 entries.
+ * local_frame = 0 evaluates the cofactors in physical coordinates exactly as the
+ * reference does; local_frame = 1 first translates each tet so that its stored vertex
+ * 0 is the origin. The element integrals are translation invariant, so both restate
+ * the same algorithm; the translated form avoids the cancellation of the absolute-
+ * coordinate cofactor sums on thin (sliver) tets.
+ * Returns 0, or a negative status (bad DOF code, degenerate tet). */
+int ned2_oracle_build(const double *nodes, int64_t Nn, const int64_t *tets, int64_t T,
+                      const int64_t *tris, int64_t F, const int64_t *edges, int64_t E,
+                      const int64_t *tet_to_field, const int64_t *tet_to_edge,
+                      const int64_t *tet_to_tri, const double *ur, const double *er,
+                      const int64_t *tetids, int64_t Ta, const int64_t *dofcodes, int ndof,
+                      double *dataE, double *dataB, int64_t *rows, int64_t *cols,
+                      int nthreads, int local_frame) {
+    if (ndof > MAXDOF || ndof <= 0) return OR_EBADCODE;
+    int64_t typ[MAXDOF], idx[MAXDOF];
+    or_parse_dofcode(dofcodes, ndof, typ, idx);
+    for (int i = 0; i < ndof; ++i)
+        if (dofcodes[i] != 64 && dofcodes[i] != 66 && dofcodes[i] != 128 && dofcodes[i] != 129)
+            return OR_EBADCODE;
+    int status = OR_OK;
+    const int64_t nd2 = (int64_t)ndof * ndof;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t ia = 0; ia < Ta; ++ia) {
+        const int64_t t = tetids ? tetids[ia] : ia;
+        const int64_t p = ia * nd2;
+        int64_t vid[4];
+        double vx[3][4];
+        for (int v = 0; v < 4; ++v) {
+            vid[v] = tets[v * T + t];
+            for (int cc = 0; cc < 3; ++cc) vx[cc][v] = nodes[cc * Nn + vid[v]];
+        }
+        if (local_frame)
+            for (int cc = 0; cc < 3; ++cc)
+                for (int v = 3; v >= 0; --v) vx[cc][v] -= vx[cc][0];
+        /* local_tet_to_triid / local_tet_to_edgeid (curlcurl.py:72-85) */
+        int64_t ltm[3][4], lem[2][6];
+        for (int s = 0; s < 4; ++s) {
+            int64_t fid = tet_to_tri[s * T + t], g3[3], o3[3];
+            for (int r = 0; r < 3; ++r) g3[r] = tris[r * F + fid];
+            or_local_map(vid, g3, 3, o3);
+            for (int r = 0; r < 3; ++r) ltm[r][s] = o3[r];
+        }
+        for (int s = 0; s < 6; ++s) {
+            int64_t eid = tet_to_edge[s * T + t], g2[2], o2[2];
+            for (int r = 0; r < 2; ++r) g2[r] = edges[r * E + eid];
+            or_local_map(vid, g2, 2, o2);
+            for (int r = 0; r < 2; ++r) lem[r][s] = o2[r];
+        }
+        cplx urt[9], ert[9], Ms[9];
+        for (int k = 0; k < 9; ++k) {
+            urt[k] = ur[2 * (k * T + t)] + I * ur[2 * (k * T + t) + 1];
+            ert[k] = er[2 * (k * T + t)] + I * er[2 * (k * T + t) + 1];
+        }
+        or_matinv(urt, Ms);
+        cplx Ke[MAXDOF * MAXDOF], Me[MAXDOF * MAXDOF];
+        int st = or_tet_stiff_mass(vx, lem, ltm, Ms, ert, dofcodes, typ, idx, ndof, Ke, Me);
+        if (st) {
+#pragma omp atomic write
+            status = st;
+            continue;
+        }
+        for (int ii = 0; ii < ndof; ++ii)
+            for (int jj = 0; jj < ndof; ++jj) {
+                int64_t e = p + ii * ndof + jj;
+                rows[e] = tet_to_field[ii * T + t];
+                cols[e] = tet_to_field[jj * T + t];
+            }
+        memcpy(dataE + 2 * p, Ke, sizeof(cplx) * nd2);
+        memcpy(dataB + 2 * p, Me, sizeof(cplx) * nd2);
+    }
+    return status;
+}
+
+/* ---------------- COO -> CSC (mth/csc_cast.py:65-177) ---------------- */
+
+static const int64_t *g_sort_rows;
+static int cmp_row_then_k(const void *pa, const void *pb) {
+    int64_t ka = *(const int64_t *)pa, kb = *(const int64_t *)pb;
+    int64_t ra = g_sort_rows[ka], rb = g_sort_rows[kb];
+    if (ra != rb) return ra < rb ? -1 : 1;
+    return ka < kb ? -1 : (ka > kb);
+}
+
+/* Phase 1 (sequential count/group) and phase 3 (indptr) of precompute_csc_pattern.
+ * Returns nnz; fills col_offsets[N+1], perm[nnz_coo] (sorted by column, then row,
+ * then COO index -- the reference's per-column sort is unstable, only the order of
+ * equal rows differs), unique counts into indptr[N+1]. */
+int64_t ned2_oracle_csc_pattern(const int64_t *rows, const int64_t *cols, int64_t nnz_coo,
+                                int64_t N, int64_t *col_offsets, int64_t *perm,
+                                int64_t *indptr) {
+    int64_t *cnt = calloc((size_t)N, sizeof(int64_t));
+    for (int64_t k = 0; k < nnz_coo; ++k) cnt[cols[k]]++;
+    col_offsets[0] = 0;
+    for (int64_t i = 0; i < N; ++i) col_offsets[i + 1] = col_offsets[i] + cnt[i];
+    int64_t *cur = malloc(sizeof(int64_t) * (size_t)N);
+    memcpy(cur, col_offsets, sizeof(int64_t) * (size_t)N);
+    for (int64_t k = 0; k < nnz_coo; ++k) perm[cur[cols[k]]++] = k;
+    g_sort_rows = rows;
+    indptr[0] = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t s = col_offsets[i], n = col_offsets[i + 1] - s;
+        if (n > 1) qsort(perm + s, (size_t)n, sizeof(int64_t), cmp_row_then_k);
+        int64_t u = n > 0 ? 1 : 0;
+        for (int64_t j = 1; j < n; ++j) u += rows[perm[s + j]] != rows[perm[s + j - 1]];
+        indptr[i + 1] = indptr[i] + u;
+    }
+    free(cnt);
+    free(cur);
+    return indptr[N];
+}
+
+/* Phase 4: indices[nnz] and target_csc[nnz_coo] (indexed by sorted position). */
+void ned2_oracle_csc_fill(const int64_t *rows, int64_t N, const int64_t *col_offsets,
+                          const int64_t *perm, const int64_t *indptr, int64_t *indices,
+                          int64_t *target) {
+    for (int64_t i = 0; i < N; ++i) {
+        int64_t s = col_offsets[i], n = col_offsets[i + 1] - s;
+        if (n == 0) continue;
+        int64_t pos = indptr[i], prev = rows[perm[s]];
+        indices[pos] = prev;
+        target[s] = pos;
+        for (int64_t j = 1; j < n; ++j) {
+            int64_t r = rows[perm[s + j]];
+            if (r != prev) {
+                indices[++pos] = r;
+                prev = r;
+            }
+            target[s + j] = pos;
+        }
+    }
+}
+
+/* scatter_to_csc_c16 (csc_cast.py:162-177): gather then per-column sum. */
+void ned2_oracle_csc_scatter(const double *data_coo, const int64_t *perm, const int64_t *target,
+                             const int64_t *col_offsets, int64_t N, double *data_csc,
+                             int64_t nnz) {
+    memset(data_csc, 0, sizeof(double) * 2 * (size_t)nnz);
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t k = col_offsets[i]; k < col_offsets[i + 1]; ++k) {
+            data_csc[2 * target[k]] += data_coo[2 * perm[k]];
+            data_csc[2 * target[k] + 1] += data_coo[2 * perm[k] + 1];
+        }
+}

@@ -71,6 +71,13 @@ _SIGS = {
     "em_ned2_assemble_a": (ctypes.c_int, [P, ctypes.POINTER(em_material_t),
                                           ctypes.POINTER(em_material_t), P, I32, P, I64, I32, P]),
     "em_ned2_rowcol": (ctypes.c_int, [P, P, P, P]),
+    "em_csc_map_create": (ctypes.c_int, [ctypes.POINTER(P), P, P]),
+    "em_csc_map_destroy": (ctypes.c_int, [P]),
+    "em_csc_map_sizes": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64),
+                                        ctypes.POINTER(I64)]),
+    "em_csc_map_export": (ctypes.c_int, [P, P, P, P, P, P, P]),
+    "em_csc_to_csc": (ctypes.c_int, [P, P, P, I32, P]),
+    "em_csc_to_csc_combine": (ctypes.c_int, [P, P, P, ctypes.c_double, P, I32, P]),
 }
 
 # every symbol include/emerge_amd.h declares

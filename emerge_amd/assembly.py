@@ -225,6 +225,81 @@ class Ned2Plan:
         return rows, cols
 
 
+class CscMap:
+    """Device CSC pattern of a plan's COO matrix (``CSCMapping``, csc_cast.py:29-63).
+
+    Built on the GPU from the DOF map (``em_csc_map_create``). ``to_csc`` sums the COO
+    values of every CSC entry on the GPU.
+    """
+
+    def __init__(self, plan: Ned2Plan, stream=None):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        check(L.em_csc_map_create(ctypes.byref(h), plan._h, _stream_ptr(stream)))
+        self._h = h
+        self.device = plan.device
+        n, nnz, nnz_coo = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(L.em_csc_map_sizes(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nnz_coo)))
+        self.N, self.nnz, self.nnz_coo = n.value, nnz.value, nnz_coo.value
+        self._cache = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().em_csc_map_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self, *names, stream=None) -> dict:
+        """Device int64 tensors among indptr, indices, perm, target_csc, col_offsets."""
+        sizes = {"indptr": self.N + 1, "indices": self.nnz, "perm": self.nnz_coo,
+                 "target_csc": self.nnz_coo, "col_offsets": self.N + 1}
+        out = {}
+        ptrs = []
+        for k in ("indptr", "indices", "perm", "target_csc", "col_offsets"):
+            if k in names:
+                t = torch.empty(sizes[k], dtype=torch.int64, device=self.device)
+                out[k] = t
+                ptrs.append(ctypes.c_void_p(t.data_ptr()))
+            else:
+                ptrs.append(None)
+        check(_lib.lib().em_csc_map_export(self._h, *ptrs, _stream_ptr(stream)))
+        return out
+
+    def to_csc(self, data: torch.Tensor, out=None, stream=None) -> torch.Tensor:
+        """CSC values of the COO values ``data`` (complex128 or complex64 on device)."""
+        prec = PREC_FP64 if data.dtype == torch.complex128 else PREC_MIXED
+        if data.dtype not in (torch.complex128, torch.complex64) or data.numel() != self.nnz_coo:
+            raise ValueError(f"COO data must be complex with {self.nnz_coo} entries")
+        data = data.contiguous()
+        if out is None:
+            out = torch.empty(self.nnz, dtype=data.dtype, device=self.device)
+        _check_out(out, self.nnz, data.dtype)
+        check(_lib.lib().em_csc_to_csc(self._h, ctypes.c_void_p(data.data_ptr()),
+                                       ctypes.c_void_p(out.data_ptr()), prec, _stream_ptr(stream)))
+        return out
+
+    def to_csc_combine(self, dataE: torch.Tensor, dataB: torch.Tensor, k0: float, out=None,
+                       stream=None) -> torch.Tensor:
+        """CSC values of ``dataE - k0**2 * dataB`` in one pass (assembler.py:494)."""
+        if dataE.dtype != dataB.dtype or dataE.numel() != dataB.numel():
+            raise ValueError("dataE and dataB must match")
+        prec = PREC_FP64 if dataE.dtype == torch.complex128 else PREC_MIXED
+        dataE, dataB = dataE.contiguous(), dataB.contiguous()
+        if out is None:
+            out = torch.empty(self.nnz, dtype=dataE.dtype, device=self.device)
+        _check_out(out, self.nnz, dataE.dtype)
+        check(_lib.lib().em_csc_to_csc_combine(self._h, ctypes.c_void_p(dataE.data_ptr()),
+                                               ctypes.c_void_p(dataB.data_ptr()), float(k0),
+                                               ctypes.c_void_p(out.data_ptr()), prec,
+                                               _stream_ptr(stream)))
+        return out
+
+
 def _check_out(t: torch.Tensor, numel: int, dt) -> None:
     if t.dtype != dt or not t.is_cuda or t.numel() != numel:
         raise ValueError(f"output must be a {dt} CUDA tensor with {numel} elements")

@@ -414,4 +414,122 @@ int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stre
     return EM_OK;
 }
 
+struct em_csc_map_s {
+    int device = 0;
+    long long N = 0, nnz_coo = 0;
+    em::CscBuild b;
+};
+
+static void free_csc(em_csc_map_s* c) {
+    if (!c) return;
+    (void)hipFree(c->b.indptr);
+    (void)hipFree(c->b.indices);
+    (void)hipFree(c->b.perm);
+    (void)hipFree(c->b.seg);
+    (void)hipFree(c->b.inc_ptr);
+    delete c;
+}
+
+int em_csc_map_create(em_csc_map_t* map, em_ned2_plan_t plan, void* stream) {
+    if (!map || !plan) {
+        set_error("csc_map_create: null argument");
+        return EM_EINVAL;
+    }
+    *map = nullptr;
+    if (plan->n_assy * 20 >= (1ll << 31)) {
+        set_error("csc_map_create: more than 2^31 tet-DOF incidences");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    auto* c = new em_csc_map_s();
+    c->device = plan->device;
+    c->N = plan->n_field;
+    c->nnz_coo = plan->n_assy * 400;
+    long long max_inc = 0;
+    hipError_t e = em::csc_build(plan->dev(), c->N, c->b, max_inc, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        delete c;
+        if (e == hipErrorInvalidValue && max_inc * 20 > 2048) {
+            set_error("csc_map_create: a DOF lies in " + std::to_string(max_inc) +
+                      " tets (at most 102 supported)");
+            return EM_EUNSUPPORTED;
+        }
+        set_error(std::string("csc_map_create: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;
+    }
+    *map = c;
+    return EM_OK;
+}
+
+int em_csc_map_destroy(em_csc_map_t map) {
+    if (!map) return EM_OK;
+    DeviceGuard g(map->device);
+    free_csc(map);
+    return EM_OK;
+}
+
+int em_csc_map_sizes(em_csc_map_t map, int64_t* n, int64_t* nnz, int64_t* nnz_coo) {
+    if (!map) {
+        set_error("null csc map");
+        return EM_EINVAL;
+    }
+    if (n) *n = map->N;
+    if (nnz) *nnz = map->b.nnz;
+    if (nnz_coo) *nnz_coo = map->nnz_coo;
+    return EM_OK;
+}
+
+int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64_t* perm,
+                      int64_t* target_csc, int64_t* col_offsets, void* stream) {
+    if (!map) {
+        set_error("null csc map");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(map->device);
+    hipStream_t s = (hipStream_t)stream;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    if (indptr) EM_HIP(hipMemcpyAsync(indptr, map->b.indptr, sizeof(int64_t) * (map->N + 1), k, s));
+    if (indices && map->b.nnz)
+        EM_HIP(hipMemcpyAsync(indices, map->b.indices, sizeof(int64_t) * map->b.nnz, k, s));
+    if (perm && map->nnz_coo)
+        EM_HIP(hipMemcpyAsync(perm, map->b.perm, sizeof(int64_t) * map->nnz_coo, k, s));
+    if (target_csc) EM_HIP(em::csc_target(map->b.seg, map->b.nnz, (long long*)target_csc, s));
+    if (col_offsets) EM_HIP(em::csc_col_offsets(map->b.inc_ptr, map->N, (long long*)col_offsets, s));
+    return EM_OK;
+}
+
+static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, void* out,
+                     int32_t precision, void* stream) {
+    if (!map) {
+        set_error("null csc map");
+        return EM_EINVAL;
+    }
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("unknown precision");
+        return EM_EINVAL;
+    }
+    if (map->b.nnz > 0 && (!E || !out)) {
+        set_error("null data");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(map->device);
+    EM_HIP(em::csc_gather(map->b.seg, map->b.perm, map->b.nnz, E, B, k0 * k0, out,
+                          precision == EM_PREC_MIXED ? 1 : 0, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_t precision,
+                  void* stream) {
+    return csc_apply(map, data_coo, nullptr, 0.0, data_csc, precision, stream);
+}
+
+int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                          void* data_csc, int32_t precision, void* stream) {
+    if (!dataB) {
+        set_error("to_csc_combine: null dataB");
+        return EM_EINVAL;
+    }
+    return csc_apply(map, dataE, dataB, k0, data_csc, precision, stream);
+}
+
 }  // extern "C"

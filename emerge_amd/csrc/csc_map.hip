@@ -1,0 +1,389 @@
+// CSC pattern of the assembled COO matrix, and the COO -> CSC value reduction.
+//
+// Reference: src/emerge/_emerge/mth/csc_cast.py:65-160 (precompute_csc_pattern) and
+// :162-177 (scatter_to_csc_c16). The reference sorts all COO (row, col) pairs; here the
+// pattern is built from the DOF map, since COO entry k = 400*ia + 20*ii + jj of
+// assembly index ia has row t2f[ii], col t2f[jj]:
+//   1. incidence lists: for every DOF d, the (ia, jj) with t2f[ia][jj] == d
+//      (histogram + exclusive scan + fill, then each list sorted so the build is
+//      deterministic);
+//   2. column c's COO entries are the 20 rows of each incidence; one wavefront ranks
+//      its (row, position) keys in LDS, flags the first entry of every distinct row and
+//      scans the flags with ballots. Pass 1 counts distinct rows (-> indptr by scan),
+//      pass 2 writes indices, perm (COO index of every entry in (col, row, k) order,
+//      the reference's fully_sorted_perm) and seg (first perm position of every CSC
+//      entry; the reference's target_csc is its inverse).
+// to_csc: each CSC entry sums its contributors data[perm[seg[q] .. seg[q+1])] in FP64
+// (complex64 inputs are accumulated in FP64 and rounded once).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "em_internal.h"
+
+namespace em {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanTile = 1024;  // 4 items per thread
+
+__global__ __launch_bounds__(kScanThreads) void scan_tiles_kernel(const long long* __restrict__ in,
+                                                                  long long* __restrict__ out,
+                                                                  long long n,
+                                                                  long long* __restrict__ sums) {
+    __shared__ long long s[kScanThreads];
+    const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * 4;
+    long long v[4], local = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        local += v[i];
+    }
+    s[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        long long t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = s[threadIdx.x] - local;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(long long* __restrict__ out,
+                                                                long long n,
+                                                                const long long* __restrict__ offs) {
+    const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * 4;
+    const long long o = offs[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (base + i < n) out[base + i] += o;
+}
+
+// Exclusive scan of n int64 values (in may equal out). Temporaries are allocated here.
+static hipError_t exclusive_scan(const long long* in, long long* out, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const long long ntiles = (n + kScanTile - 1) / kScanTile;
+    long long *sums = nullptr, *offs = nullptr;
+    hipError_t e = hipMalloc((void**)&sums, sizeof(long long) * ntiles);
+    if (e != hipSuccess) return e;
+    e = hipMalloc((void**)&offs, sizeof(long long) * ntiles);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, in,
+                           out, n, sums);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && ntiles > 1) {
+        e = exclusive_scan(sums, offs, ntiles, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s,
+                               out, n, offs);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // temporaries are freed below
+    (void)hipFree(sums);
+    (void)hipFree(offs);
+    return e;
+}
+
+__device__ __forceinline__ long long tet_of(const int* tetids, long long ia) {
+    return tetids ? (long long)tetids[ia] : ia;
+}
+
+__global__ __launch_bounds__(256) void inc_count_kernel(const int* __restrict__ t2f,
+                                                        const int* __restrict__ tetids,
+                                                        long long n_assy,
+                                                        unsigned long long* __restrict__ cnt) {
+    const long long n = n_assy * 20;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long ia = e / 20;
+        const int d = t2f[tet_of(tetids, ia) * 20 + (int)(e - ia * 20)];
+        atomicAdd(cnt + d, 1ull);
+    }
+}
+
+__global__ __launch_bounds__(256) void inc_fill_kernel(const int* __restrict__ t2f,
+                                                       const int* __restrict__ tetids,
+                                                       long long n_assy,
+                                                       const long long* __restrict__ inc_ptr,
+                                                       unsigned long long* __restrict__ cursor,
+                                                       int* __restrict__ inc) {
+    const long long n = n_assy * 20;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long ia = e / 20;
+        const int d = t2f[tet_of(tetids, ia) * 20 + (int)(e - ia * 20)];
+        const unsigned long long pos = atomicAdd(cursor + d, 1ull);
+        inc[inc_ptr[d] + (long long)pos] = (int)e;
+    }
+}
+
+// Sort every incidence list (short) ascending; also record the longest list.
+__global__ __launch_bounds__(256) void inc_sort_kernel(const long long* __restrict__ inc_ptr,
+                                                       int* __restrict__ inc, long long N,
+                                                       unsigned long long* __restrict__ maxlen) {
+    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= N) return;
+    const long long s = inc_ptr[d], n = inc_ptr[d + 1] - s;
+    int* a = inc + s;
+    for (long long i = 1; i < n; ++i) {
+        const int v = a[i];
+        long long j = i - 1;
+        while (j >= 0 && a[j] > v) {
+            a[j + 1] = a[j];
+            --j;
+        }
+        a[j + 1] = v;
+    }
+    atomicMax(maxlen, (unsigned long long)n);
+}
+
+constexpr int kColCap = 2048;  // COO entries per column handled in LDS (102 incidences)
+
+template <bool FILL>
+__global__ __launch_bounds__(64) void csc_column_kernel(
+    const int* __restrict__ t2f, const int* __restrict__ tetids,
+    const long long* __restrict__ inc_ptr, const int* __restrict__ inc, long long N,
+    long long* __restrict__ ucount, const long long* __restrict__ indptr,
+    long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg) {
+    __shared__ unsigned long long key[kColCap];
+    __shared__ unsigned long long srt[kColCap];
+    const int lane = threadIdx.x;
+    for (long long c = blockIdx.x; c < N; c += gridDim.x) {
+        const long long s = inc_ptr[c];
+        const int n = (int)(20 * (inc_ptr[c + 1] - s));
+        for (int i = lane; i < n; i += 64) {
+            const int e = inc[s + i / 20];
+            const long long ia = e / 20;
+            const int row = t2f[tet_of(tetids, ia) * 20 + (i % 20)];
+            key[i] = ((unsigned long long)(unsigned)row << 32) | (unsigned)i;
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) {
+            const unsigned long long ki = key[i];
+            int r = 0;
+            for (int j = 0; j < n; ++j) r += key[j] < ki;
+            srt[r] = ki;
+        }
+        __syncthreads();
+        long long carry = 0;
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + lane;
+            const bool valid = i < n;
+            const unsigned long long cur = valid ? srt[i] : 0ull;
+            const bool flag = valid && (i == 0 || (cur >> 32) != (srt[i - 1] >> 32));
+            const unsigned long long mask = __ballot(flag);
+            if (FILL && valid) {
+                const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+                const long long q = indptr[c] + carry + __popcll(mask & upto) - 1;
+                const int li = (int)(cur & 0xffffffffu);
+                const int e = inc[s + li / 20];
+                const long long ia = e / 20;
+                const int jj = e % 20, ii = li % 20;
+                perm[20 * s + i] = 400 * ia + 20 * ii + jj;
+                if (flag) {
+                    indices[q] = (long long)(cur >> 32);
+                    seg[q] = 20 * s + i;
+                }
+            }
+            carry += __popcll(mask);
+        }
+        if (!FILL && lane == 0) ucount[c] = carry;
+        __syncthreads();  // LDS reuse by the next column
+    }
+}
+
+template <typename In>
+struct LoadC;
+template <>
+struct LoadC<double2> {
+    __device__ static double2 get(const double2* p, long long k) { return p[k]; }
+};
+template <>
+struct LoadC<float2> {
+    __device__ static double2 get(const float2* p, long long k) {
+        const float2 v = p[k];
+        return make_double2((double)v.x, (double)v.y);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ void store_c(T* p, long long q, double2 v);
+template <>
+__device__ __forceinline__ void store_c<double2>(double2* p, long long q, double2 v) {
+    p[q] = v;
+}
+template <>
+__device__ __forceinline__ void store_c<float2>(float2* p, long long q, double2 v) {
+    p[q] = make_float2((float)v.x, (float)v.y);
+}
+
+// data_csc[q] = sum over contributors of (E[k] - k0sq * B[k]) (B may be null: E[k]).
+template <typename T>
+__global__ __launch_bounds__(256) void csc_gather_kernel(const long long* __restrict__ seg,
+                                                         const long long* __restrict__ perm,
+                                                         long long nnz, const T* __restrict__ E,
+                                                         const T* __restrict__ B, double k0sq,
+                                                         T* __restrict__ out) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long j0 = seg[q], j1 = seg[q + 1];
+        double2 acc = make_double2(0.0, 0.0);
+        for (long long j = j0; j < j1; ++j) {
+            const long long k = perm[j];
+            double2 v = LoadC<T>::get(E, k);
+            if (B) {
+                const double2 b = LoadC<T>::get(B, k);
+                v.x -= k0sq * b.x;
+                v.y -= k0sq * b.y;
+            }
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        store_c<T>(out, q, acc);
+    }
+}
+
+__global__ __launch_bounds__(256) void seg_to_target_kernel(const long long* __restrict__ seg,
+                                                            long long nnz,
+                                                            long long* __restrict__ target) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (long long)gridDim.x * blockDim.x)
+        for (long long j = seg[q]; j < seg[q + 1]; ++j) target[j] = q;
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(const long long* __restrict__ in, long long n,
+                                                    long long mul, long long* __restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = in[i] * mul;
+}
+
+static unsigned grid_for(long long n, long long cap = 65536 * 4) {
+    long long b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// Builds the pattern of plan mesh m (N DOFs). On success the caller owns b's buffers.
+hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
+                     hipStream_t s) {
+    const long long nnz_coo = m.n_assy * 400;
+    hipError_t e;
+    unsigned long long *cnt = nullptr, *maxlen = nullptr;
+    int* inc = nullptr;
+    long long* ucount = nullptr;
+#define TRY(x)                          \
+    do {                                \
+        e = (x);                        \
+        if (e != hipSuccess) goto fail; \
+    } while (0)
+    TRY(hipMalloc((void**)&cnt, sizeof(unsigned long long) * (N + 1)));
+    TRY(hipMalloc((void**)&b.inc_ptr, sizeof(long long) * (N + 1)));
+    TRY(hipMalloc((void**)&maxlen, sizeof(unsigned long long)));
+    TRY(hipMalloc((void**)&inc, sizeof(int) * (size_t)(m.n_assy * 20 > 0 ? m.n_assy * 20 : 1)));
+    TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
+    TRY(hipMemsetAsync(maxlen, 0, sizeof(unsigned long long), s));
+    if (m.n_assy > 0) {
+        hipLaunchKernelGGL(inc_count_kernel, dim3(grid_for(m.n_assy * 20)), dim3(256), 0, s, m.t2f,
+                           m.tetids, m.n_assy, cnt);
+        TRY(hipGetLastError());
+    }
+    TRY(exclusive_scan((const long long*)cnt, b.inc_ptr, N + 1, s));
+    TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
+    if (m.n_assy > 0) {
+        hipLaunchKernelGGL(inc_fill_kernel, dim3(grid_for(m.n_assy * 20)), dim3(256), 0, s, m.t2f,
+                           m.tetids, m.n_assy, b.inc_ptr, cnt, inc);
+        TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(inc_sort_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+                       b.inc_ptr, inc, N, maxlen);
+    TRY(hipGetLastError());
+    {
+        unsigned long long ml = 0;
+        TRY(hipMemcpyAsync(&ml, maxlen, sizeof(ml), hipMemcpyDeviceToHost, s));
+        TRY(hipStreamSynchronize(s));
+        max_inc = (long long)ml;
+        if (20 * max_inc > kColCap) {
+            e = hipErrorInvalidValue;
+            goto fail;
+        }
+    }
+    // pass 1: distinct rows per column
+    ucount = reinterpret_cast<long long*>(cnt);  // reuse: N+1 int64
+    TRY(hipMemsetAsync(ucount, 0, sizeof(long long) * (N + 1), s));
+    {
+        const unsigned g = (unsigned)(N < (1 << 20) ? N : (1 << 20));
+        if (N > 0) {
+            hipLaunchKernelGGL(csc_column_kernel<false>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
+                               b.inc_ptr, inc, N, ucount, nullptr, nullptr, nullptr, nullptr);
+            TRY(hipGetLastError());
+        }
+        TRY(hipMalloc((void**)&b.indptr, sizeof(long long) * (N + 1)));
+        TRY(exclusive_scan(ucount, b.indptr, N + 1, s));
+        TRY(hipMemcpyAsync(&b.nnz, b.indptr + N, sizeof(long long), hipMemcpyDeviceToHost, s));
+        TRY(hipStreamSynchronize(s));
+        TRY(hipMalloc((void**)&b.indices, sizeof(long long) * (b.nnz > 0 ? b.nnz : 1)));
+        TRY(hipMalloc((void**)&b.seg, sizeof(long long) * (b.nnz + 1)));
+        TRY(hipMalloc((void**)&b.perm, sizeof(long long) * (nnz_coo > 0 ? nnz_coo : 1)));
+        TRY(hipMemcpyAsync(b.seg + b.nnz, &nnz_coo, sizeof(long long), hipMemcpyHostToDevice, s));
+        if (N > 0) {
+            hipLaunchKernelGGL(csc_column_kernel<true>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
+                               b.inc_ptr, inc, N, nullptr, b.indptr, b.indices, b.perm, b.seg);
+            TRY(hipGetLastError());
+        }
+        TRY(hipStreamSynchronize(s));
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(maxlen);
+    (void)hipFree(inc);
+    return hipSuccess;
+fail:
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(cnt);
+    (void)hipFree(maxlen);
+    (void)hipFree(inc);
+    (void)hipFree(b.indptr);
+    (void)hipFree(b.indices);
+    (void)hipFree(b.perm);
+    (void)hipFree(b.seg);
+    (void)hipFree(b.inc_ptr);
+    b = CscBuild();
+    return e;
+#undef TRY
+}
+
+hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz, const void* E,
+                      const void* B, double k0sq, void* out, int complex64, hipStream_t s) {
+    if (nnz == 0) return hipSuccess;
+    if (complex64)
+        hipLaunchKernelGGL(csc_gather_kernel<float2>, dim3(grid_for(nnz)), dim3(256), 0, s, seg,
+                           perm, nnz, (const float2*)E, (const float2*)B, k0sq, (float2*)out);
+    else
+        hipLaunchKernelGGL(csc_gather_kernel<double2>, dim3(grid_for(nnz)), dim3(256), 0, s, seg,
+                           perm, nnz, (const double2*)E, (const double2*)B, k0sq, (double2*)out);
+    return hipGetLastError();
+}
+
+hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s) {
+    if (nnz == 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_to_target_kernel, dim3(grid_for(nnz)), dim3(256), 0, s, seg, nnz, target);
+    return hipGetLastError();
+}
+
+hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(N + 1)), dim3(256), 0, s, inc_ptr, N + 1, 20ll,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace em

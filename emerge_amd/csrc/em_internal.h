@@ -44,6 +44,22 @@ hipError_t launch_check_tables(const int4* tets, long long T, const long long* e
 hipError_t launch_check_volumes(const double4* xyz, const int4* tets, long long T,
                                 unsigned long long* bad, hipStream_t s);
 
+// CSC map (csc_map.hip)
+struct CscBuild {
+    long long* indptr = nullptr;   // [N+1]
+    long long* indices = nullptr;  // [nnz]
+    long long* perm = nullptr;     // [nnz_coo]
+    long long* seg = nullptr;      // [nnz+1]
+    long long* inc_ptr = nullptr;  // [N+1]
+    long long nnz = 0;
+};
+hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
+                     hipStream_t s);
+hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz, const void* E,
+                      const void* B, double k0sq, void* out, int complex64, hipStream_t s);
+hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s);
+hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s);
+
 void set_error(const std::string& msg);
 
 }  // namespace em

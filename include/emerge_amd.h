@@ -119,6 +119,36 @@ int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er,
 /* The static COO pattern: rows/cols int64 [400 * Ta] (curlcurl.py:449-453). */
 int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stream);
 
+/*
+ * CSC map of a plan's COO pattern (CSCMapping.from_rowcol -> precompute_csc_pattern,
+ * csc_cast.py:51-53, 65-160), built on the device from the DOF map. Rows inside a
+ * column are sorted ascending with duplicates merged; duplicates are summed in COO
+ * index order. Blocks until the build is done (a one-time cost per mesh).
+ * Fails with EM_EUNSUPPORTED if some DOF lies in more than 102 tets.
+ */
+int em_csc_map_create(em_csc_map_t* map, em_ned2_plan_t plan, void* stream);
+int em_csc_map_destroy(em_csc_map_t map);
+/* N (columns), nnz (CSC entries), nnz_coo (COO entries) */
+int em_csc_map_sizes(em_csc_map_t map, int64_t* n, int64_t* nnz, int64_t* nnz_coo);
+/*
+ * Copy the map's arrays to caller DEVICE buffers (any may be NULL):
+ * indptr [N+1], indices [nnz], perm [nnz_coo] (COO index of each entry in sorted
+ * order, csc_cast.py fully_sorted_perm), target_csc [nnz_coo] (CSC position of each
+ * sorted entry), col_offsets [N+1] (first sorted entry of each column).
+ */
+int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64_t* perm,
+                      int64_t* target_csc, int64_t* col_offsets, void* stream);
+/*
+ * CSCMapping.to_csc (csc_cast.py:55-63, 162-177): data_csc[q] = sum of the COO values
+ * of entry q. complex128 in/out (EM_PREC_FP64) or complex64 in/out (EM_PREC_MIXED,
+ * summed in FP64 and rounded once). Device pointers.
+ */
+int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_t precision,
+                  void* stream);
+/* Fused to_csc(dataE - k0^2 * dataB) (assembler.py:494) without forming the COO sum. */
+int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                          void* data_csc, int32_t precision, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,8 +47,18 @@ def _load():
     lib.ned2_oracle_csc_fill.argtypes = [P, i64, P, P, P, P, P]
     lib.ned2_oracle_csc_scatter.restype = None
     lib.ned2_oracle_csc_scatter.argtypes = [P, P, P, P, i64, P, i64]
+    lib.ned2_oracle_keast.restype = None
+    lib.ned2_oracle_keast.argtypes = [P]
     _lib = lib
     return lib
+
+
+def keast_table() -> np.ndarray:
+    """The oracle's 11-point rule (5, 11): weights, then barycentric coordinates
+    (curlcurl.py:277-283)."""
+    out = np.empty((5, 11), np.float64)
+    _load().ned2_oracle_keast(_ptr(out))
+    return out
 
 
 def _ptr(a: np.ndarray):

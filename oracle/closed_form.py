@@ -135,12 +135,13 @@ def _bilinear(fa: VPoly, fb: VPoly, mat, absV):
     return acc
 
 
-def element_matrices(verts, gids, eps, mu):
+def element_matrices(verts, gids, eps, mu, mu_is_inverse=False):
     """Exact (K, M) 20x20 of one tet, reference slot order.
 
     verts: 4 vertex coordinates (rationals) in stored order; gids: their global ids;
-    eps, mu: 3x3 rational (or complex-rational pairs are not supported) tensors.
-    The reference uses K with mu^-1 (mth/optimized.py:315 matinv) and M with eps.
+    eps, mu: 3x3 rational tensors (complex tensors: call once per real part, the forms
+    are linear in eps and mu^-1). The reference uses K with mu^-1
+    (mth/optimized.py:315 matinv) and M with eps; ``mu_is_inverse`` passes mu^-1.
     """
     v = [[Fraction(c) for c in p] for p in verts]
     J = [[v[k][r] - v[0][r] for k in (1, 2, 3)] for r in range(3)]
@@ -156,7 +157,8 @@ def element_matrices(verts, gids, eps, mu):
         gl[k] = [Jinv[k - 1][c] for c in range(3)]
     gl[0] = [-(gl[1][c] + gl[2][c] + gl[3][c]) for c in range(3)]
     g = [[s * x for x in gl[n]] for n in range(4)]   # reference: s * lambda
-    muinv = _inv3([[Fraction(x) for x in row] for row in mu])
+    muq = [[Fraction(x) for x in row] for row in mu]
+    muinv = muq if mu_is_inverse else _inv3(muq)
     epsq = [[Fraction(x) for x in row] for row in eps]
     fs, cs = [], []
     for d, code in enumerate(CODES):

@@ -413,3 +413,6 @@ void ned2_oracle_csc_scatter(const double *data_coo, const int64_t *perm, const 
             data_csc[2 * target[k] + 1] += data_coo[2 * perm[k] + 1];
         }
 }
+
+/* Copy of the quadrature table (weights row, then 4 barycentric rows), for tests. */
+void ned2_oracle_keast(double *out) { memcpy(out, DPTS, sizeof(DPTS)); }

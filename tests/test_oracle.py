@@ -1,0 +1,306 @@
+"""CPU checks that pin the oracle (the C restatement of curlcurl.py:222-456) against
+results fixed by mathematics: exact rational integrals, the quadrature rule's exactness,
+finite differences, element invariants, the Whitney-gradient null space and the
+analytic eigenfrequencies of a PEC rectangular cavity."""
+from fractions import Fraction
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import closed_form as cf
+from tests.conftest import ref_tensor, rel_fro
+from emerge_amd.mesh import Nedelec2Field, TetMesh, hex_box_mesh, delaunay_box_mesh
+
+C0 = 299792458.0
+
+
+def _exact(verts, gids, eps, muinv):
+    """Exact (K, M) as complex arrays; K is linear in mu^-1 and M in eps, so complex
+    tensors are split into real and imaginary parts (each rational)."""
+    K = np.zeros((20, 20), complex)
+    M = np.zeros((20, 20), complex)
+    I3 = np.eye(3).tolist()
+    for unit, part in ((1.0, np.real), (1j, np.imag)):
+        e, mi = part(np.asarray(eps, complex)), part(np.asarray(muinv, complex))
+        fe = [[Fraction(float(x)) for x in row] for row in e]
+        fm = [[Fraction(float(x)) for x in row] for row in mi]
+        Kq, Mq = cf.element_matrices(verts, gids, fe if np.any(e) else I3,
+                                     fm if np.any(mi) else I3, mu_is_inverse=True)
+        if np.any(mi):
+            K += unit * np.array([[float(x) for x in r] for r in Kq])
+        if np.any(e):
+            M += unit * np.array([[float(x) for x in r] for r in Mq])
+    return K, M
+
+
+# ---------------------------------------------------------------------------
+# quadrature
+# ---------------------------------------------------------------------------
+
+def test_keast_rule_exact_to_degree_4():
+    """sum_n w_n lambda^alpha(x_n) = alpha! 3! / (|alpha|+3)! for all |alpha| <= 4."""
+    from math import factorial
+    D = oracle.keast_table()
+    w, L = D[0], D[1:]
+    assert abs(w.sum() - 1.0) < 1e-15
+    assert np.allclose(L.sum(axis=0), 1.0, atol=1e-15)
+    worst = 0.0
+    for deg in range(5):
+        for alpha in itertools.product(range(deg + 1), repeat=4):
+            if sum(alpha) != deg:
+                continue
+            val = float(np.sum(w * np.prod([L[n] ** alpha[n] for n in range(4)], axis=0)))
+            ex = np.prod([factorial(a) for a in alpha]) * 6 / factorial(deg + 3)
+            worst = max(worst, abs(val - ex) / ex)
+    assert worst < 1e-14
+    # and it is not exact at degree 5 (so the check above has teeth)
+    val5 = float(np.sum(w * L[0] ** 5))
+    assert abs(val5 - 120 * 6 / factorial(8)) > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# closed form vs the oracle
+# ---------------------------------------------------------------------------
+
+def _one_tet_field(verts, order):
+    nodes = np.asarray(verts, float).T.copy()
+    tets = np.asarray(order, np.int64).reshape(4, 1)
+    return Nedelec2Field(TetMesh(nodes, tets))
+
+
+@pytest.mark.parametrize("mat", ["identity", "aniso_complex", "full_nonsym"])
+def test_oracle_equals_exact_integrals_all_vertex_orders(mat):
+    base = [[0, 0, 0], [3, 1, 0], [1, 4, 1], [0, 1, 5]]
+    rng = np.random.default_rng(1)
+    if mat == "identity":
+        eps, mu = np.eye(3), np.eye(3)
+    elif mat == "aniso_complex":
+        eps = np.diag([2 - 0.1j, 3 - 0.05j, 5 + 0j])
+        mu = np.diag([1 + 0j, 2 - 0.2j, 4 + 0j])
+    else:
+        eps = np.round(rng.normal(size=(3, 3)) * 8) / 8 + 3 * np.eye(3) + 0.25j * np.eye(3)
+        mu = np.round(rng.normal(size=(3, 3)) * 8) / 8 + 4 * np.eye(3)
+    muinv = np.linalg.inv(mu)
+    worst = 0.0
+    for order in itertools.permutations(range(4)):
+        field = _one_tet_field(base, order)
+        er = eps.reshape(3, 3, 1).astype(complex)
+        ur = mu.reshape(3, 3, 1).astype(complex)
+        dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(field, er, ur, np.array([]))
+        verts = [[Fraction(c) for c in base[o]] for o in order]
+        K, M = _exact(verts, list(order), eps, muinv)
+        worst = max(worst, rel_fro(dE.reshape(20, 20), K), rel_fro(dB.reshape(20, 20), M))
+        # COO layout of femdata.py:57-73 / curlcurl.py:449-453
+        t2f = field.tet_to_field[:, 0]
+        assert np.array_equal(rows, np.repeat(t2f, 20))
+        assert np.array_equal(cols, np.tile(t2f, 20))
+    assert worst < 1e-13
+
+
+def test_negative_orientation_flips_odd_functions():
+    """|V| normalisation (optimized.py:142-167): swapping two stored vertices maps each
+    block entry to a slot permutation times s = -1 on (code 64, odd code) pairs."""
+    base = np.array([[0, 0, 0], [1, 0.2, 0], [0.1, 1, 0.3], [0.2, 0.1, 1.1]])
+    f1 = _one_tet_field(base, [0, 1, 2, 3])
+    # same vertices, positions 2 and 3 swapped in storage (ids keep their coordinates)
+    nodes = base.T.copy()
+    f2 = Nedelec2Field(TetMesh(nodes, np.array([[0], [1], [3], [2]], np.int64)))
+    I = np.eye(3).reshape(3, 3, 1).astype(complex)
+    K1 = oracle.tet_mass_stiffness_matrices(f1, I, I, np.array([]))[0].reshape(20, 20)
+    K2 = oracle.tet_mass_stiffness_matrices(f2, I, I, np.array([]))[0].reshape(20, 20)
+    d1, d2 = f1.tet_to_field[:, 0], f2.tet_to_field[:, 0]
+    pos = {d: i for i, d in enumerate(d2)}
+    P = np.array([pos[d] for d in d1])
+    K2p = K2[np.ix_(P, P)]
+    s = np.where(np.arange(20) < 6, 1.0, -1.0)
+    assert np.allclose(K2p, K1 * np.outer(s, s), rtol=0, atol=1e-12 * np.abs(K1).max())
+    assert not np.allclose(K2p, K1, rtol=0, atol=1e-6 * np.abs(K1).max())
+
+
+def test_oracle_hex_assembly_equals_exact():
+    mesh = hex_box_mesh(2.0, 1.0, 1.0, 2, 1, 1)
+    field = Nedelec2Field(mesh)
+    T = field.n_tets
+    I = ref_tensor(T, "identity", None)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(field, I, I, np.array([]))
+    nodes_q = [[Fraction(float(x)) for x in row] for row in mesh.nodes]
+    Kx, Mx = cf.assemble_dense(nodes_q, mesh.tets, field.tet_to_field, field.n_field)
+    import scipy.sparse as sp
+    N = field.n_field
+    Ko = sp.coo_matrix((dE, (rows, cols)), shape=(N, N)).toarray()
+    Mo = sp.coo_matrix((dB, (rows, cols)), shape=(N, N)).toarray()
+    Ke = np.zeros((N, N))
+    Me = np.zeros((N, N))
+    for (r, c), v in Kx.items():
+        Ke[r, c] = float(v)
+    for (r, c), v in Mx.items():
+        Me[r, c] = float(v)
+    assert rel_fro(Ko, Ke) < 1e-14 and rel_fro(Mo, Me) < 1e-14
+
+
+def test_curl_machinery_against_finite_differences():
+    """The closed form's symbolic curl equals a central finite difference of the field."""
+    verts = np.array([[0.1, 0.0, 0.2], [1.0, 0.3, 0.1], [0.2, 1.1, 0.0], [0.3, 0.2, 0.9]])
+    J = (verts[1:] - verts[0]).T
+    Jinv = np.linalg.inv(J)
+    gl = [None] * 4
+    for k in (1, 2, 3):
+        gl[k] = [Fraction(float(x)) for x in Jinv[k - 1]]
+    gl[0] = [-(gl[1][c] + gl[2][c] + gl[3][c]) for c in range(3)]
+
+    def lam(x):
+        xi = Jinv @ (x - verts[0])
+        return np.array([1 - xi.sum(), *xi])
+
+    def evalp(poly, x):
+        l = lam(x)
+        out = np.zeros(3)
+        for m, v in poly.t.items():
+            out += np.prod(l ** np.array(m)) * np.array([float(c) for c in v])
+        return out
+
+    x0 = verts.mean(axis=0) + np.array([0.01, -0.02, 0.015])
+    h = 1e-5
+    for code, (i, j, k) in [(64, (0, 2, 0)), (66, (1, 3, 0)), (128, (0, 1, 3)), (129, (1, 2, 3))]:
+        f, _ = cf.basis(code, i, j, k, gl)
+        c = evalp(f.curl(gl), x0)
+        D = np.zeros((3, 3))
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            D[:, a] = (evalp(f, x0 + e) - evalp(f, x0 - e)) / (2 * h)
+        fd = np.array([D[2, 1] - D[1, 2], D[0, 2] - D[2, 0], D[1, 0] - D[0, 1]])
+        assert np.allclose(c, fd, rtol=1e-6, atol=1e-8), (code, c, fd)
+
+
+# ---------------------------------------------------------------------------
+# sliver conditioning of the reference arithmetic
+# ---------------------------------------------------------------------------
+
+def test_local_frame_oracle_matches_exact_on_slivers():
+    """On the thinnest Delaunay tets the reference's absolute-coordinate cofactor sums
+    lose accuracy; the translated (local-frame) evaluation of the same algorithm stays
+    at rounding level against the exact integral."""
+    field = Nedelec2Field(delaunay_box_mesh(2500, seed=11))
+    m = field.mesh
+    T = field.n_tets
+    I = ref_tensor(T, "identity", None)
+    dE_abs = oracle.tet_mass_stiffness_matrices(field, I, I, [], local_frame=False)[0]
+    dE_loc = oracle.tet_mass_stiffness_matrices(field, I, I, [], local_frame=True)[0]
+    dets = np.abs(m.signed_dets())
+    worst_abs, worst_loc = 0.0, 0.0
+    for t in np.argsort(dets)[:6]:
+        gids = m.tets[:, t]
+        verts = [[Fraction(float(c)) for c in m.nodes[:, g]] for g in gids]
+        K, _ = cf.element_matrices(verts, list(gids), np.eye(3).tolist(), np.eye(3).tolist())
+        Kx = np.array([[float(x) for x in r] for r in K])
+        worst_loc = max(worst_loc, rel_fro(dE_loc[400 * t:400 * t + 400].reshape(20, 20), Kx))
+        worst_abs = max(worst_abs, rel_fro(dE_abs[400 * t:400 * t + 400].reshape(20, 20), Kx))
+    assert worst_loc < 1e-13
+    assert worst_abs < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# invariants
+# ---------------------------------------------------------------------------
+
+def test_element_invariants():
+    field = Nedelec2Field(delaunay_box_mesh(200, seed=2))
+    T = field.n_tets
+    rng = np.random.default_rng(0)
+    er = ref_tensor(T, "diag", rng, complex_=False)
+    ur = ref_tensor(T, "identity", rng)
+    dE, dB, _, _ = oracle.tet_mass_stiffness_matrices(field, er, ur, [], local_frame=True)
+    for t in range(0, T, max(1, T // 20)):
+        K = dE[400 * t:400 * t + 400].reshape(20, 20)
+        M = dB[400 * t:400 * t + 400].reshape(20, 20)
+        assert np.array_equal(K, K.T) and np.array_equal(M, M.T)
+        sk = np.linalg.svd(K.real, compute_uv=False)
+        assert np.sum(sk > 1e-9 * sk[0]) == 11
+        ev = np.linalg.eigvalsh(M.real)
+        assert ev.min() > 0
+
+
+def test_whitney_gradients_in_stiffness_null_space(wr90_field):
+    """K g = 0 for g[code-64 DOF of edge (a<b)] = phi_b - phi_a, since grad L_n =
+    sum_j (L_j grad L_n - L_n grad L_j): checks DOF map and edge orientation globally."""
+    import scipy.sparse as sp
+    f = wr90_field
+    m = f.mesh
+    T = f.n_tets
+    I = ref_tensor(T, "identity", None)
+    dE, _, rows, cols = oracle.tet_mass_stiffness_matrices(f, I, I, [])
+    N = f.n_field
+    K = sp.coo_matrix((dE.real, (rows, cols)), shape=(N, N)).tocsr()
+    phi = np.random.default_rng(4).normal(size=m.n_nodes)
+    g = np.zeros(N)
+    g[:m.n_edges] = phi[m.edges[1]] - phi[m.edges[0]]
+    r = K @ g
+    assert np.linalg.norm(r) < 1e-10 * abs(K).sum() / N * np.linalg.norm(g)
+    # a wrong orientation on one edge breaks it
+    g[7] = -g[7]
+    assert np.linalg.norm(K @ g) > 1e-3 * np.linalg.norm(r + 1e-300) + 1e-6
+
+
+def _pec_interior_dofs(field):
+    m = field.mesh
+    E, F = m.n_edges, m.n_tris
+    cnt = np.bincount(m.tet_to_tri.ravel(), minlength=F)
+    bf = np.nonzero(cnt == 1)[0]
+    tri = m.tris[:, bf]
+    Nn = m.n_nodes
+    ekey = m.edges[0] * Nn + m.edges[1]
+    be = np.unique(np.concatenate([np.searchsorted(ekey, tri[a] * Nn + tri[b])
+                                   for a, b in ((0, 1), (0, 2), (1, 2))]))
+    bdofs = np.concatenate([be, be + E, bf + 2 * E, bf + 2 * E + F])
+    keep = np.ones(field.n_field, bool)
+    keep[bdofs] = False
+    return np.nonzero(keep)[0]
+
+
+def test_pec_cavity_eigenfrequencies(wr90_field):
+    """22.86 x 10.16 x 50 mm PEC cavity: TE101, TE102, TE103 from the analytic formula
+    f = c/2 sqrt((m/a)^2 + (n/b)^2 + (p/d)^2), through oracle assembly, the CSC map
+    restatement and PEC elimination (solver.py:2229-2230 style)."""
+    import scipy.sparse.linalg as sla
+    f = wr90_field
+    T = f.n_tets
+    I = ref_tensor(T, "identity", None)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, I, I, [])
+    cm = oracle.CSCMap(rows, cols, f.n_field)
+    K = cm.to_csc(dE).real
+    M = cm.to_csc(dB).real
+    keep = _pec_interior_dofs(f)
+    Ki = K[keep][:, keep].tocsc()
+    Mi = M[keep][:, keep].tocsc()
+    sigma = (2 * np.pi * 9e9 / C0) ** 2
+    vals = sla.eigsh(Ki, k=3, M=Mi, sigma=sigma, which="LM", return_eigenvectors=False)
+    fr = np.sort(np.sqrt(vals) * C0 / (2 * np.pi))
+    a, d = 22.86e-3, 50e-3
+    ex = np.array([C0 / 2 * np.hypot(1 / a, p / d) for p in (1, 2, 3)])
+    assert np.allclose(ex / 1e9, [7.20997, 8.88517, 11.13032], atol=5e-5)
+    assert np.all(np.abs(fr - ex) / ex < 2e-3), (fr, ex)
+
+
+# ---------------------------------------------------------------------------
+# CSC restatement vs SciPy
+# ---------------------------------------------------------------------------
+
+def test_oracle_csc_matches_scipy(wr90_field):
+    import scipy.sparse as sp
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(6)
+    er = ref_tensor(T, "scalar", rng)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, er, ref_tensor(T, "identity", rng), [])
+    N = f.n_field
+    cm = oracle.CSCMap(rows, cols, N)
+    A = cm.to_csc(dE - 43925.66356 * dB)
+    S = sp.coo_matrix((dE - 43925.66356 * dB, (rows, cols)), shape=(N, N)).tocsc()
+    S.sum_duplicates()
+    S.sort_indices()
+    assert np.array_equal(A.indptr, S.indptr) and np.array_equal(A.indices, S.indices)
+    assert rel_fro(A.data, S.data) < 1e-14
+    assert cm.nnz == 2_039_716  # the waveguide mesh's pattern size

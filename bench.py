@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Nedelec-2 curl-curl system assembly throughput on one MI355X.
+
+Workload: unit-box Delaunay mesh of 400k seeded uniform points plus a jittered boundary
+grid (about 2.6M tets), per-tet complex eps_r = Re(1 - j tan_d) with Re ~ U[1,10],
+tan_d ~ U[0,0.02], per-tet diagonal mu_r ~ U[1,2]^3, f = 10 GHz. One step assembles the
+COO values of A = K - k0^2 M for every tet in one kernel launch; inputs are resident in
+HBM. The headline ``value`` is the mixed-precision kernel (FP64 geometry, FP32
+contraction, complex64 output); the plain FP64 kernel (complex128 output) and the
+relative Frobenius error of mixed against FP64 are reported beside it, with the CPU
+restatement (oracle) timed on a bounded sample of the same mesh.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Nédélec tets assembled/s + HBM GB/s: mixed-FP32 vs FP64 at ≤1e-6 rel err"
+HBM_PEAK_GBS = 8000.0
+C0 = 299792458.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bytes_per_tet(out_bytes_per_entry: int, nodes_per_tet: float) -> float:
+    """Algorithmic HBM bytes per tet of the fused A kernel.
+
+    node ids 16 (4 x int32) + node coordinates 24 * Nn/T (3 x f64, each node read once)
+    + eps_r 16 (one complex128 per tet) + mu_r 48 (3 complex128 per tet, diagonal)
+    + 400 output entries.
+    """
+    return 16 + 24 * nodes_per_tet + 16 + 48 + 400 * out_bytes_per_entry
+
+
+def time_launches(fn, steps: int, warmup: int):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def rel_err_chunked(a32, a64, chunk=1 << 26) -> float:
+    import torch
+    num = 0.0
+    den = 0.0
+    for s in range(0, a64.numel(), chunk):
+        x = a64[s:s + chunk]
+        d = a32[s:s + chunk].to(torch.complex128) - x
+        num += float(torch.sum(d.real ** 2 + d.imag ** 2))
+        den += float(torch.sum(x.real ** 2 + x.imag ** 2))
+    return (num / den) ** 0.5
+
+
+def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int):
+    """The oracle (C/OpenMP restatement of _matrix_builder, K and M plus int64 rows/cols
+    like the reference) on the first ``chunk`` tets of the mesh, repeated for about
+    ``seconds``; returns tets/s and the thread count."""
+    import numpy as np
+    import oracle
+    nthreads = min(16, os.cpu_count() or 1)
+    m = field.mesh
+    ids = np.arange(min(chunk, field.n_tets), dtype=np.int64)
+    args = (m.nodes, m.tets, m.tris, m.edges, field.tet_to_field, m.tet_to_edge, m.tet_to_tri,
+            ur_ref, er_ref, ids, field.dofcodes3d)
+    oracle.matrix_builder(*args, nthreads=nthreads)  # warm
+    done, t0 = 0, time.perf_counter()
+    while True:
+        oracle.matrix_builder(*args, nthreads=nthreads)
+        done += ids.shape[0]
+        el = time.perf_counter() - t0
+        if el >= seconds and done >= 3 * ids.shape[0]:
+            break
+    return done / el, nthreads, done, el
+
+
+def load_traffic(workload_key: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and
+    calibrated on a copy kernel; committed under profiles/ by the profiling recipe."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        data = json.load(fh)
+    entry = data.get(workload_key)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=400_000)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-chunk", type=int, default=50_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 2 GiB device copy, so that a rocprofv3 --pmc "
+                         "run can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the assembly shards by tet range with no exchange "
+              "between GPUs, and only the single-GPU figure is reported here "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world}).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    mesh = delaunay_box_mesh(args.points, seed=args.seed)
+    field = Nedelec2Field(mesh)
+    T, Nn = field.n_tets, mesh.n_nodes
+    mesh_s = time.perf_counter() - t0
+    log(f"mesh: {T} tets, {Nn} nodes, {field.n_field} DOFs in {mesh_s:.1f}s")
+
+    rng = np.random.default_rng(args.seed + 1)
+    eps_np = rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T))
+    mu_np = rng.uniform(1, 2, (T, 3)).astype(np.complex128)
+    plan = Ned2Plan.from_field(field)
+    eps = plan.material(torch.from_numpy(eps_np))
+    mu = plan.material(torch.from_numpy(mu_np))
+    k0 = 2 * np.pi * 10e9 / C0
+    nnz = plan.nnz_coo
+    log(f"plan ready ({time.perf_counter() - t0:.1f}s); COO entries {nnz}")
+
+    a32 = torch.empty(nnz, dtype=torch.complex64, device="cuda")
+    ms32, wall32 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32),
+                                 args.steps, args.warmup)
+    log(f"mixed: {ms32:.3f} ms/launch")
+    res64 = None
+    err = None
+    if not args.no_fp64:
+        a64 = torch.empty(nnz, dtype=torch.complex128, device="cuda")
+        ms64, wall64 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "fp64", out=a64),
+                                     args.steps, args.warmup)
+        log(f"fp64: {ms64:.3f} ms/launch")
+        plan.assemble_a(eps, mu, k0, "mixed", out=a32)
+        torch.cuda.synchronize()
+        err = rel_err_chunked(a32, a64)
+        log(f"mixed vs fp64 relative Frobenius error {err:.3e}")
+        b64 = bytes_per_tet(16, Nn / T)
+        res64 = {"tets_per_s": T / (ms64 * 1e-3), "ms_per_step": ms64,
+                 "gb_per_s": T * b64 / (ms64 * 1e-3) / 1e9,
+                 "roofline_frac": T * b64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 "bytes_per_tet": b64, "dtype": "f64", "output": "complex128 COO"}
+        del a64
+
+    if args.copy_calibration:
+        src = torch.ones(1 << 31, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 2 GiB device copy done")
+
+    b32 = bytes_per_tet(8, Nn / T)
+    achieved = T * b32 / (ms32 * 1e-3) / 1e9
+    workload_key = f"delaunay_p{args.points}_s{args.seed}_mixed"
+    traffic = load_traffic(workload_key)
+
+    cpu = None
+    if not args.no_cpu:
+        er_ref = np.zeros((3, 3, T), np.complex128)
+        ur_ref = np.zeros((3, 3, T), np.complex128)
+        for d in range(3):
+            er_ref[d, d] = eps_np
+            ur_ref[d, d] = mu_np[:, d]
+        log("cpu baseline (oracle) ...")
+        tps, nth, done, el = cpu_baseline(field, er_ref, ur_ref, args.cpu_seconds, args.cpu_chunk)
+        cpu = {"value": tps, "unit": "tets/s", "cores": nth, "kind": "port",
+               "sample": f"first {min(args.cpu_chunk, T)} tets of the same mesh, repeated "
+                         f"({done} tets in {el:.1f}s); K and M complex128 + int64 rows/cols "
+                         f"as the reference's _matrix_builder"}
+        log(f"cpu: {tps:.0f} tets/s on {nth} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": T / (ms32 * 1e-3),
+        "unit": "tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Delaunay unit box, 400k seeded uniform points + jittered "
+                               "boundary grid; A = K - k0^2 M COO complex64, f = 10 GHz; "
+                               "per-tet complex scalar eps_r, diagonal mu_r",
+                   "n_tets": T, "n_nodes": Nn, "n_dofs": field.n_field, "nnz_coo": nnz,
+                   "points": args.points, "seed": args.seed,
+                   "precision": "FP64 geometry, FP32 contraction, complex64 output"},
+        "hbm_gb_per_s": achieved,
+        "rel_err_vs_fp64": err,
+        "wall_s_timed_region": wall32,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_tet": b32, "kernel": "ned2_assemble_kernel<float, MODE_A>"},
+        "fp64": res64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -24,7 +24,7 @@ _lib = None
 
 
 def build() -> str:
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
 
 

@@ -64,6 +64,8 @@ _SIGS = {
     "em_ned2_plan_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_int, P, I64, P, I64, P, I64,
                                            P, I64, P, P, P, I64, P, I64, P, I32]),
     "em_ned2_plan_destroy": (ctypes.c_int, [P]),
+    "em_ned2_plan_set_tuning": (ctypes.c_int, [P, I32]),
+    "em_ned2_plan_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
     "em_ned2_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64),
                                           ctypes.POINTER(I64)]),
     "em_ned2_assemble_km": (ctypes.c_int, [P, ctypes.POINTER(em_material_t),
@@ -78,6 +80,11 @@ _SIGS = {
     "em_csc_map_export": (ctypes.c_int, [P, P, P, P, P, P, P]),
     "em_csc_to_csc": (ctypes.c_int, [P, P, P, I32, P]),
     "em_csc_to_csc_combine": (ctypes.c_int, [P, P, P, ctypes.c_double, P, I32, P]),
+    "em_ned2_backward": (ctypes.c_int, [P, P, ctypes.c_double, I32, P, I32, P, P]),
+    "em_graph_begin_capture": (ctypes.c_int, [P]),
+    "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
+    "em_graph_launch": (ctypes.c_int, [P, P]),
+    "em_graph_destroy": (ctypes.c_int, [P]),
 }
 
 # every symbol include/emerge_amd.h declares

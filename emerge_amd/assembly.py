@@ -148,6 +148,8 @@ class Ned2Plan:
         self.n_assy = n_assy
         self.n_field = int(n_field)
         self.nnz_coo = 400 * n_assy
+        self.tet_offsets = np.array([0, self.n_tets], np.int64)
+        self.dof_offsets = np.array([0, self.n_field], np.int64)
 
     @classmethod
     def from_field(cls, field, tetids=None, device=None) -> "Ned2Plan":
@@ -155,6 +157,51 @@ class Ned2Plan:
         return cls(m.nodes, m.tets, m.tris, m.edges, field.get_tet_to_field(), m.tet_to_edge,
                    m.tet_to_tri, field.n_field, tetids=tetids, dofcodes=field.dofcodes3d,
                    device=device)
+
+    @classmethod
+    def from_fields(cls, fields, device=None) -> "Ned2Plan":
+        """One plan over several meshes (a batch of uneven size), assembled in one launch.
+
+        The meshes are concatenated with their node, edge, face and DOF numbers offset,
+        which keeps every mesh's vertex order (so every element block is unchanged) and
+        makes the global system block diagonal. ``plan.tet_offsets[i]`` is the first
+        assembly index of mesh i (its COO values start at 400 * that), and
+        ``plan.dof_offsets[i]`` the first global DOF of mesh i.
+        """
+        nodes, tets, tris, edges, t2f, t2e, t2t = [], [], [], [], [], [], []
+        nn = ne = nf_ = ndof = 0
+        toffs, doffs = [0], [0]
+        for f in fields:
+            m = f.mesh
+            nodes.append(m.nodes)
+            tets.append(m.tets + nn)
+            tris.append(m.tris + nn)
+            edges.append(m.edges + nn)
+            t2e.append(m.tet_to_edge + ne)
+            t2t.append(m.tet_to_tri + nf_)
+            t2f.append(f.get_tet_to_field() + ndof)
+            nn += m.n_nodes
+            ne += m.n_edges
+            nf_ += m.n_tris
+            ndof += f.n_field
+            toffs.append(toffs[-1] + m.n_tets)
+            doffs.append(ndof)
+        plan = cls(np.concatenate(nodes, axis=1), np.concatenate(tets, axis=1),
+                   np.concatenate(tris, axis=1), np.concatenate(edges, axis=1),
+                   np.concatenate(t2f, axis=1), np.concatenate(t2e, axis=1),
+                   np.concatenate(t2t, axis=1), ndof, device=device)
+        plan.tet_offsets = np.asarray(toffs, np.int64)
+        plan.dof_offsets = np.asarray(doffs, np.int64)
+        return plan
+
+    def set_tuning(self, nb: int) -> None:
+        """Tets per LDS sub-batch of the assembly kernels (1, 2 or 4)."""
+        check(_lib.lib().em_ned2_plan_set_tuning(self._h, int(nb)))
+
+    def get_tuning(self) -> int:
+        v = ctypes.c_int32()
+        check(_lib.lib().em_ned2_plan_get_tuning(self._h, ctypes.byref(v)))
+        return v.value
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -216,6 +263,27 @@ class Ned2Plan:
         if scalar and out.dim() == 2:
             return out[0]
         return out
+
+    def backward(self, grad_coo: torch.Tensor, k0: float, eps_kind: int | None, grad_eps,
+                 mu_kind: int | None, grad_mu_inv, stream=None) -> None:
+        """dL/d eps_r and dL/d mu_r^-1 (tet-major complex128 buffers, written in place)
+        from dL/dA of the COO values of ``assemble_a`` at wavenumber ``k0``."""
+        if grad_coo.dtype != torch.complex128 or grad_coo.numel() != self.nnz_coo:
+            raise ValueError(f"upstream gradient must be complex128 with {self.nnz_coo} entries")
+        grad_coo = grad_coo.contiguous()
+        width = {_lib.EM_MAT_SCALAR: 1, _lib.EM_MAT_DIAG: 3, _lib.EM_MAT_FULL: 9}
+        ptrs = []
+        for kind, g in ((eps_kind, grad_eps), (mu_kind, grad_mu_inv)):
+            if g is None:
+                ptrs.append(None)
+                continue
+            if g.dtype != torch.complex128 or not g.is_contiguous() or \
+                    g.numel() != self.n_tets * width[kind]:
+                raise ValueError("gradient buffers must be contiguous complex128 of the leaf size")
+            ptrs.append(ctypes.c_void_p(g.data_ptr()))
+        check(_lib.lib().em_ned2_backward(self._h, ctypes.c_void_p(grad_coo.data_ptr()), float(k0),
+                                          int(eps_kind or 0), ptrs[0], int(mu_kind or 0), ptrs[1],
+                                          _stream_ptr(stream)))
 
     def rowcol(self, stream=None):
         rows = torch.empty(self.nnz_coo, dtype=torch.int64, device=self.device)

@@ -30,8 +30,7 @@ struct em_ned2_plan_s {
     int4* tets = nullptr;
     int* t2f = nullptr;
     int* tetids = nullptr;
-    double* k0sq = nullptr;  // device scratch for wavenumbers
-    int k0cap = 0;
+    int nb = 2;  // tets per LDS sub-batch of the assembly kernel (autotuned)
 
     em::MeshDev dev() const {
         em::MeshDev m;
@@ -85,7 +84,6 @@ void free_plan(em_ned2_plan_s* p) {
     (void)hipFree(p->tets);
     (void)hipFree(p->t2f);
     (void)hipFree(p->tetids);
-    (void)hipFree(p->k0sq);
     delete p;
 }
 
@@ -330,10 +328,9 @@ int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er, const em_m
         return EM_EINVAL;
     }
     DeviceGuard g(plan->device);
-    const double zero = 0.0;
     EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_KM,
-                               precision, needs_anti(er, ur), nullptr, &zero, 0, dataE, dataB, 0,
-                               0, (hipStream_t)stream));
+                               precision, needs_anti(er, ur), nullptr, 0, dataE, dataB, 0,
+                               plan->nb, (hipStream_t)stream));
     return EM_OK;
 }
 
@@ -376,27 +373,107 @@ int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er, const em_ma
         k0sq[f] = k0[f] * k0[f];
     }
     DeviceGuard g(plan->device);
-    hipStream_t s = (hipStream_t)stream;
-    if (n_k0 == 1) {
-        EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_A,
-                                   precision, needs_anti(er, ur), nullptr, k0sq.data(), 1, out,
-                                   nullptr, 0, 0, s));
-        return EM_OK;
+    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true),
+                               n_k0 == 1 ? em::MODE_A : em::MODE_AF, precision,
+                               needs_anti(er, ur), k0sq.data(), n_k0, out, nullptr, freq_stride,
+                               plan->nb, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_ned2_backward(em_ned2_plan_t plan, const void* grad_coo, double k0, int32_t eps_kind,
+                     void* grad_eps, int32_t mu_kind, void* grad_mu_inv, void* stream) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
     }
-    if (plan->k0cap < n_k0) {
-        EM_HIP(hipStreamSynchronize(s));
-        (void)hipFree(plan->k0sq);
-        plan->k0sq = nullptr;
-        plan->k0cap = 0;
-        if ((st = dmalloc(&plan->k0sq, (size_t)n_k0))) return st;
-        plan->k0cap = n_k0;
+    if (plan->n_assy > 0 && !grad_coo) {
+        set_error("backward: null upstream gradient");
+        return EM_EINVAL;
     }
-    EM_HIP(hipMemcpyAsync(plan->k0sq, k0sq.data(), sizeof(double) * n_k0, hipMemcpyHostToDevice, s));
-    // the host vector must outlive the copy
-    EM_HIP(hipStreamSynchronize(s));
-    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_AF,
-                               precision, needs_anti(er, ur), plan->k0sq, k0sq.data(), n_k0, out,
-                               nullptr, freq_stride, 0, s));
+    if ((grad_eps && (eps_kind < EM_MAT_SCALAR || eps_kind > EM_MAT_FULL)) ||
+        (grad_mu_inv && (mu_kind < EM_MAT_SCALAR || mu_kind > EM_MAT_FULL))) {
+        set_error("backward: unknown leaf kind");
+        return EM_EINVAL;
+    }
+    if (!std::isfinite(k0)) {
+        set_error("backward: non-finite k0");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    EM_HIP(em::launch_backward(plan->dev(), (const double2*)grad_coo, k0 * k0, eps_kind, mu_kind,
+                               (double2*)grad_eps, (double2*)grad_mu_inv, (hipStream_t)stream));
+    return EM_OK;
+}
+
+struct em_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int em_graph_begin_capture(void* stream) {
+    if (!stream) {
+        set_error("graph capture needs a non-default stream");
+        return EM_EINVAL;
+    }
+    EM_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return EM_OK;
+}
+
+int em_graph_end_capture(void* stream, em_graph_t* graph) {
+    if (!stream || !graph) {
+        set_error("null argument");
+        return EM_EINVAL;
+    }
+    *graph = nullptr;
+    auto* gr = new em_graph_s();
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &gr->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        delete gr;
+        set_error(std::string("graph capture: ") + hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    *graph = gr;
+    return EM_OK;
+}
+
+int em_graph_launch(em_graph_t graph, void* stream) {
+    if (!graph) {
+        set_error("null graph");
+        return EM_EINVAL;
+    }
+    EM_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_graph_destroy(em_graph_t graph) {
+    if (!graph) return EM_OK;
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return EM_OK;
+}
+
+int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t nb) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    if (nb != 1 && nb != 2 && nb != 4) {
+        set_error("tuning: tets per sub-batch must be 1, 2 or 4");
+        return EM_EINVAL;
+    }
+    plan->nb = nb;
+    return EM_OK;
+}
+
+int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* nb) {
+    if (!plan || !nb) {
+        set_error("null argument");
+        return EM_EINVAL;
+    }
+    *nb = plan->nb;
     return EM_OK;
 }
 

@@ -31,11 +31,19 @@ struct MeshDev {
 
 enum OutMode { MODE_A = 0, MODE_KM = 1, MODE_AF = 2 };
 
+// Wavenumbers of one frequency-sweep launch, passed by value as a kernel argument.
+constexpr int kMaxK0 = 128;
+struct K0Pack {
+    double v[kMaxK0];  // k0^2
+};
+
 // Launchers (defined in ned2_assemble.hip). All return hipError_t of the launch.
+// k0sq: host array of nf values of k0^2 (MODE_A uses k0sq[0]; MODE_KM ignores it).
+// nb: tets per LDS sub-batch (1, 2 or 4; other values mean 2).
 hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
-                           int mode, int precision, int anti, const double* k0sq_dev,
-                           const double* k0sq_host, int nf, void* out0, void* out1,
-                           long long freq_stride, int nb, hipStream_t stream);
+                           int mode, int precision, int anti, const double* k0sq, int nf,
+                           void* out0, void* out1, long long freq_stride, int nb,
+                           hipStream_t stream);
 hipError_t launch_rowcol(const MeshDev& m, long long* rows, long long* cols, hipStream_t s);
 hipError_t launch_check_tables(const int4* tets, long long T, const long long* edges,
                                long long E, const long long* tris, long long F,
@@ -43,6 +51,10 @@ hipError_t launch_check_tables(const int4* tets, long long T, const long long* e
                                unsigned long long* bad, hipStream_t s);
 hipError_t launch_check_volumes(const double4* xyz, const int4* tets, long long T,
                                 unsigned long long* bad, hipStream_t s);
+
+// Backward of the fused assembly w.r.t. eps_r and mu_r^-1 (ned2_backward.hip).
+hipError_t launch_backward(const MeshDev& m, const double2* grad, double k0sq, int eps_kind,
+                           int mu_kind, double2* g_eps, double2* g_mu, hipStream_t stream);
 
 // CSC map (csc_map.hip)
 struct CscBuild {

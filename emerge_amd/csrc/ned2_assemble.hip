@@ -138,14 +138,14 @@ __device__ __forceinline__ void congruence(const double A[3][3], const Cd E[9], 
         }
 }
 
-// LDS record of one tet: symmetric (6) and antisymmetric (3) parts of G_K and G_M,
-// the rank code and the orientation sign.
-template <typename R>
+// LDS record of one tet: symmetric (6) parts of G_K and G_M, their antisymmetric (3)
+// parts when FULL, the rank code and the orientation sign.
+template <typename R, bool FULL>
 struct TileRec {
     R gk[kTile][6][2];
     R gm[kTile][6][2];
-    R ak[kTile][3][2];
-    R am[kTile][3][2];
+    R ak[FULL ? kTile : 1][3][2];
+    R am[FULL ? kTile : 1][3][2];
     int code[kTile];
     R sgn[kTile];
 };
@@ -170,11 +170,45 @@ __device__ __forceinline__ void split_store(const Cd G[9], R (*sym)[2], R (*anti
     }
 }
 
-// Phase 1 for one tet (FP64 throughout).
+// Diagonal (or scalar) tensor of one tet as 3 complex values.
+__device__ __forceinline__ void load_diag(const MatDesc& m, long long t, Cd e[3]) {
+    const double2* p = m.p + t * m.ts;
+    if (m.kind == EM_MAT_SCALAR) {
+        const double2 v = p[0];
+        e[0] = e[1] = e[2] = {v.x, v.y};
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double2 v = p[d * m.cs];
+            e[d] = {v.x, v.y};
+        }
+    }
+}
+
+// Symmetric components (00, 11, 22, 01, 02, 12) of A diag(e) A^T * scale.
 template <typename R>
+__device__ __forceinline__ void diag_congruence_store(const double A[3][3], const Cd e[3],
+                                                      double scale, R (*sym)[2]) {
+    constexpr int P[6] = {0, 1, 2, 0, 0, 1}, Q[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double w = A[P[c]][k] * A[Q[c]][k];
+            re += w * e[k].re;
+            im += w * e[k].im;
+        }
+        sym[c][0] = (R)(re * scale);
+        sym[c][1] = (R)(im * scale);
+    }
+}
+
+// Phase 1 for one tet (FP64 throughout).
+template <typename R, bool FULL>
 __device__ __forceinline__ void tet_geometry(const MeshDev& m, const MatDesc& eps,
                                              const MatDesc& mu, long long ia, int lt,
-                                             TileRec<R>& rec) {
+                                             TileRec<R, FULL>& rec) {
     const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
     const int4 nd = m.tets[t];
     const int n[4] = {nd.x, nd.y, nd.z, nd.w};
@@ -182,52 +216,54 @@ __device__ __forceinline__ void tet_geometry(const MeshDev& m, const MatDesc& ep
 #pragma unroll
     for (int a = 0; a < 4; ++a)
         r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
-    double4 X[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) X[a] = m.xyz[n[a]];
-    double W[4][3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        double4 v = (r[0] == k) ? X[0] : (r[1] == k) ? X[1] : (r[2] == k) ? X[2] : X[3];
-        W[k][0] = v.x;
-        W[k][1] = v.y;
-        W[k][2] = v.z;
-    }
-    double Jc[3][3];  // Jc[q] = column q = w_{q+1} - w_0
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) Jc[q][c] = W[q + 1][c] - W[0][c];
-    double C[3][3];  // rows: J1 x J2, J2 x J0, J0 x J1
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-        const double* u = Jc[(p + 1) % 3];
-        const double* v = Jc[(p + 2) % 3];
-        C[p][0] = u[1] * v[2] - u[2] * v[1];
-        C[p][1] = u[2] * v[0] - u[0] * v[2];
-        C[p][2] = u[0] * v[1] - u[1] * v[0];
-    }
+    // canonical vertices: the node ids in ascending order (5-comparator network), so
+    // the coordinates are gathered already in canonical order
+    const int s0 = min(n[0], n[1]), s1 = max(n[0], n[1]);
+    const int s2 = min(n[2], n[3]), s3 = max(n[2], n[3]);
+    const int w0 = min(s0, s2), t2 = max(s0, s2);
+    const int t1 = min(s1, s3), w3 = max(s1, s3);
+    const int w1 = min(t1, t2), w2 = max(t1, t2);
+    const double4 X0 = m.xyz[w0], X1 = m.xyz[w1], X2 = m.xyz[w2], X3 = m.xyz[w3];
+    // Jc[q] = column q of J = w_{q+1} - w_0
+    const double Jc[3][3] = {{X1.x - X0.x, X1.y - X0.y, X1.z - X0.z},
+                             {X2.x - X0.x, X2.y - X0.y, X2.z - X0.z},
+                             {X3.x - X0.x, X3.y - X0.y, X3.z - X0.z}};
+    // rows of C = det(J) J^-1: J1 x J2, J2 x J0, J0 x J1
+    const double C[3][3] = {
+        {Jc[1][1] * Jc[2][2] - Jc[1][2] * Jc[2][1], Jc[1][2] * Jc[2][0] - Jc[1][0] * Jc[2][2],
+         Jc[1][0] * Jc[2][1] - Jc[1][1] * Jc[2][0]},
+        {Jc[2][1] * Jc[0][2] - Jc[2][2] * Jc[0][1], Jc[2][2] * Jc[0][0] - Jc[2][0] * Jc[0][2],
+         Jc[2][0] * Jc[0][1] - Jc[2][1] * Jc[0][0]},
+        {Jc[0][1] * Jc[1][2] - Jc[0][2] * Jc[1][1], Jc[0][2] * Jc[1][0] - Jc[0][0] * Jc[1][2],
+         Jc[0][0] * Jc[1][1] - Jc[0][1] * Jc[1][0]}};
     const double det = Jc[0][0] * C[0][0] + Jc[0][1] * C[0][1] + Jc[0][2] * C[0][2];
-    const double adet = fabs(det);
+    const double inv_adet = 1.0 / fabs(det);
     const int code = r[0] * 16 + r[1] * 4 + r[2];
     const int inv = (r[0] > r[1]) + (r[0] > r[2]) + (r[0] > r[3]) + (r[1] > r[2]) +
                     (r[1] > r[3]) + (r[2] > r[3]);
     const double s = ((det < 0.0) != (inv & 1)) ? -1.0 : 1.0;
 
-    Cd E[9], G[9];
-    load_tensor(eps, t, E);
-    congruence(C, E, eps.kind, 1.0 / adet, G);
-    split_store<R>(G, rec.gm[lt], rec.am[lt]);
-
-    load_tensor(mu, t, E);
-    if (mu.invert) invert_tensor(E);
-    double JT[3][3];  // rows of J^T = columns of J
+    if constexpr (!FULL) {
+        Cd e[3];
+        load_diag(eps, t, e);
+        diag_congruence_store<R>(C, e, inv_adet, rec.gm[lt]);
+        load_diag(mu, t, e);
+        if (mu.invert) {
+            const Cd one = {1.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) JT[q][c] = Jc[q][c];
-    congruence(JT, E, mu.kind, 1.0 / adet, G);
-    split_store<R>(G, rec.gk[lt], rec.ak[lt]);
+            for (int d = 0; d < 3; ++d) e[d] = cdiv(one, e[d]);
+        }
+        diag_congruence_store<R>(Jc, e, inv_adet, rec.gk[lt]);
+    } else {
+        Cd E[9], G[9];
+        load_tensor(eps, t, E);
+        congruence(C, E, eps.kind, inv_adet, G);
+        split_store<R>(G, rec.gm[lt], rec.am[lt]);
+        load_tensor(mu, t, E);
+        if (mu.invert) invert_tensor(E);
+        congruence(Jc, E, mu.kind, inv_adet, G);
+        split_store<R>(G, rec.gk[lt], rec.ak[lt]);
+    }
     rec.code[lt] = code;
     rec.sgn[lt] = (R)s;
 }
@@ -261,15 +297,15 @@ __device__ __forceinline__ void stream_out(const void* lds, void* gbl, int bytes
 template <typename R, int MODE, bool ANTI, int NB>
 struct Smem {
     static constexpr int kStageMats = (MODE == MODE_A) ? 1 : 2;
-    TileRec<R> rec;
+    TileRec<R, ANTI> rec;
     alignas(16) C2<R> stage[2][kStageMats][NB * 400];
     unsigned char slot[64][20];
 };
 
 template <typename R, int MODE, bool ANTI, int NB>
 __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
-    MeshDev m, MatDesc eps, MatDesc mu, const double* __restrict__ k0sq, int nf,
-    double k0sq0, void* __restrict__ out0, void* __restrict__ out1, long long freq_stride) {
+    MeshDev m, MatDesc eps, MatDesc mu, K0Pack k0p, int nf, void* __restrict__ out0,
+    void* __restrict__ out1, long long freq_stride) {
     using O = typename OutVec<R>::T;
     __shared__ Smem<R, MODE, ANTI, NB> sm;
     constexpr int NS = Smem<R, MODE, ANTI, NB>::kStageMats;
@@ -278,7 +314,8 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
 
     const long long tile0 = (long long)blockIdx.x * kTile;
     const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
-    if (threadIdx.x < ntile) tet_geometry<R>(m, eps, mu, tile0 + threadIdx.x, threadIdx.x, sm.rec);
+    if (threadIdx.x < ntile)
+        tet_geometry<R, ANTI>(m, eps, mu, tile0 + threadIdx.x, threadIdx.x, sm.rec);
 
     // Pair coefficients of this thread (registers).
     const int j = threadIdx.x;
@@ -297,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
         qm[c] = ANTI ? (R)TANTI_M[pj][c] : (R)0;
     }
     const bool aodd = a >= 6, bodd = b >= 6;
-    const R kq = (R)k0sq0;
+    const R kq = (R)k0p.v[0];
     __syncthreads();
 
     const int nsub = (ntile + NB - 1) / NB;
@@ -320,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
                         mr += tm[c] * sm.rec.gm[lt][c][0];
                         mi += tm[c] * sm.rec.gm[lt][c][1];
                     }
-                    if (ANTI) {
+                    if constexpr (ANTI) {
                         R xr = 0, xi = 0, yr = 0, yi = 0;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
@@ -368,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
             constexpr int per = 16 / (int)sizeof(O);  // complex entries per store
             const int nvec = nent / per;
             for (int f = 0; f < nf; ++f) {
-                const R kf = (R)k0sq[f];
+                const R kf = (R)k0p.v[f];
                 u32x4* dst = reinterpret_cast<u32x4*>(reinterpret_cast<O*>(out0) +
                                                       (long long)f * freq_stride + ia0 * 400);
                 for (int i = threadIdx.x; i < nvec; i += kBlock) {
@@ -392,54 +429,81 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
     }
 }
 
+// Launch over all tiles; MODE_AF launches once per chunk of up to kMaxK0 wavenumbers
+// (passed by value, so the launch needs no device buffer and can be graph-captured).
 template <typename R, int MODE, bool ANTI, int NB>
 static hipError_t launch_t(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
-                           const double* k0sq_dev, int nf, double k0sq0, void* out0, void* out1,
+                           const double* k0sq, int nf, void* out0, void* out1,
                            long long freq_stride, hipStream_t stream) {
+    using O = typename OutVec<R>::T;
     const long long ntiles = (m.n_assy + kTile - 1) / kTile;
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, ANTI, NB>), dim3((unsigned)ntiles),
-                       dim3(kBlock), 0, stream, m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1,
-                       freq_stride);
-    return hipGetLastError();
+    const int nchunk = (MODE == MODE_AF) ? nf : 1;
+    for (int f0 = 0; f0 < nchunk; f0 += kMaxK0) {
+        K0Pack p;
+        const int n = (MODE == MODE_AF) ? min(kMaxK0, nf - f0) : 1;
+        for (int f = 0; f < kMaxK0; ++f) p.v[f] = (k0sq && f < n) ? k0sq[f0 + f] : 0.0;
+        void* o0 = reinterpret_cast<O*>(out0) + (long long)f0 * freq_stride;
+        hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, ANTI, NB>), dim3((unsigned)ntiles),
+                           dim3(kBlock), 0, stream, m, eps, mu, p, n, o0, out1, freq_stride);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename R, int MODE, bool FULL>
+static hipError_t launch_nb(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
+                            const double* k0sq, int nf, void* out0, void* out1,
+                            long long freq_stride, int nb, hipStream_t stream) {
+    // MODE_KM stages two matrices per tet: it keeps the smallest LDS footprint.
+    if constexpr (MODE == MODE_KM) {
+        return launch_t<R, MODE, FULL, 1>(m, eps, mu, k0sq, nf, out0, out1,
+                                          freq_stride, stream);
+    }
+    if (nb == 1)
+        return launch_t<R, MODE, FULL, 1>(m, eps, mu, k0sq, nf, out0, out1,
+                                          freq_stride, stream);
+    if (nb == 4)
+        return launch_t<R, MODE, FULL, 4>(m, eps, mu, k0sq, nf, out0, out1,
+                                          freq_stride, stream);
+    return launch_t<R, MODE, FULL, 2>(m, eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                      stream);
 }
 
 template <typename R, int MODE>
 static hipError_t launch_anti(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int anti,
-                              const double* k0sq_dev, int nf, double k0sq0, void* out0,
-                              void* out1, long long freq_stride, int nb, hipStream_t stream) {
-    (void)nb;
+                              const double* k0sq, int nf, void* out0, void* out1,
+                              long long freq_stride, int nb, hipStream_t stream) {
     if (anti)
-        return launch_t<R, MODE, true, 2>(m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1,
-                                          freq_stride, stream);
-    return launch_t<R, MODE, false, 2>(m, eps, mu, k0sq_dev, nf, k0sq0, out0, out1, freq_stride,
-                                       stream);
+        return launch_nb<R, MODE, true>(m, eps, mu, k0sq, nf, out0, out1,
+                                        freq_stride, nb, stream);
+    return launch_nb<R, MODE, false>(m, eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                     nb, stream);
 }
 
 template <typename R>
 static hipError_t launch_mode(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
-                              int anti, const double* k0sq_dev, const double* k0sq_host, int nf,
-                              void* out0, void* out1, long long freq_stride, int nb,
-                              hipStream_t stream) {
+                              int anti, const double* k0sq, int nf, void* out0, void* out1,
+                              long long freq_stride, int nb, hipStream_t stream) {
     if (mode == MODE_A)
-        return launch_anti<R, MODE_A>(m, eps, mu, anti, k0sq_dev, nf, k0sq_host[0], out0, out1,
-                                      freq_stride, nb, stream);
+        return launch_anti<R, MODE_A>(m, eps, mu, anti, k0sq, 1, out0, out1, freq_stride, nb,
+                                      stream);
     if (mode == MODE_KM)
-        return launch_anti<R, MODE_KM>(m, eps, mu, anti, k0sq_dev, nf, 0.0, out0, out1,
-                                       freq_stride, nb, stream);
-    return launch_anti<R, MODE_AF>(m, eps, mu, anti, k0sq_dev, nf, 0.0, out0, out1, freq_stride,
-                                   nb, stream);
+        return launch_anti<R, MODE_KM>(m, eps, mu, anti, nullptr, 0, out0, out1, freq_stride, nb,
+                                       stream);
+    return launch_anti<R, MODE_AF>(m, eps, mu, anti, k0sq, nf, out0, out1, freq_stride, nb,
+                                   stream);
 }
 
 hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
-                           int precision, int anti, const double* k0sq_dev,
-                           const double* k0sq_host, int nf, void* out0, void* out1,
-                           long long freq_stride, int nb, hipStream_t stream) {
+                           int precision, int anti, const double* k0sq, int nf, void* out0,
+                           void* out1, long long freq_stride, int nb, hipStream_t stream) {
     if (precision == EM_PREC_FP64)
-        return launch_mode<double>(m, eps, mu, mode, anti, k0sq_dev, k0sq_host, nf, out0, out1,
-                                   freq_stride, nb, stream);
-    return launch_mode<float>(m, eps, mu, mode, anti, k0sq_dev, k0sq_host, nf, out0, out1,
-                              freq_stride, nb, stream);
+        return launch_mode<double>(m, eps, mu, mode, anti, k0sq, nf, out0, out1, freq_stride, nb,
+                                   stream);
+    return launch_mode<float>(m, eps, mu, mode, anti, k0sq, nf, out0, out1, freq_stride, nb,
+                              stream);
 }
 
 // ---------------------------------------------------------------------------

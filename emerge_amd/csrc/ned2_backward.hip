@@ -1,0 +1,279 @@
+// Backward of the fused assembly A = K(mu_r^-1) - k0^2 M(eps_r) with respect to the
+// per-tet material tensors (FP64).
+//
+// The reference has no gradients (its optimiser is derivative-free); this kernel is the
+// reverse of ned2_assemble_kernel. A is linear in eps_r and mu_r^-1, so the gradient
+// depends only on the geometry and the upstream gradient dL/dA of the COO values
+// (PyTorch's convention for complex tensors: dL/dRe + i dL/dIm). With the forward's
+// notation (ned2_assemble.hip): each reference entry (ii, jj) of a tet equals
+//   sg * (sum_c gk_c TSYM_K[p][c] + sa sum_c ak_c TANTI_K[p][c]
+//         - k0^2 (sum_c gm_c TSYM_M[p][c] + sa sum_c am_c TANTI_M[p][c]))
+// for its canonical pair p, so with w = sg * (g(ii,jj) + g(jj,ii)) (g(ii,ii) once)
+//   dL/dgk_c = sum_p w TSYM_K[p][c],  dL/dgm_c = -k0^2 sum_p w TSYM_M[p][c]
+// (and the antisymmetric parts with sa). Those sums are wavefront shuffle reductions
+// over the 210 pair-threads. Then, from G_M = C eps C^T / |det J| and
+// G_K = Jc mu^-1 Jc^T / |det J| (Jc rows = columns of J):
+//   dL/deps = C^T W_M C / |det J|,   dL/dmu^-1 = Jc^T W_K Jc / |det J|
+// reduced to the leaf's kind (scalar: trace, diagonal: diagonal, full: all 9).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "em_internal.h"
+#include "ned2_tables.inc"
+
+namespace em {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 64;
+
+template <bool FULL>
+struct NComp {
+    // per matrix: 6 symmetric (+3 antisymmetric) complex components, as doubles
+    static constexpr int per = FULL ? 18 : 12;
+    static constexpr int total = 2 * per;
+};
+
+template <bool FULL, int NB>
+struct BwdSmem {
+    double Jc[kTile][9];
+    double C[kTile][9];
+    double iad[kTile];
+    double sgn[kTile];
+    long long tet[kTile];
+    int code[kTile];
+    alignas(16) double2 G[NB][400];
+    double part[4][NB][NComp<FULL>::total];
+    unsigned char slot[64][20];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// g_out (tet-major, complex) for a 3x3 complex gradient Gr (row-major) reduced to kind.
+__device__ __forceinline__ void write_leaf(double2* out, long long t, int kind,
+                                          const double Gre[9], const double Gim[9]) {
+    if (!out) return;
+    if (kind == EM_MAT_SCALAR) {
+        out[t] = make_double2(Gre[0] + Gre[4] + Gre[8], Gim[0] + Gim[4] + Gim[8]);
+    } else if (kind == EM_MAT_DIAG) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[3 * t + d] = make_double2(Gre[4 * d], Gim[4 * d]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[9 * t + i] = make_double2(Gre[i], Gim[i]);
+    }
+}
+
+// dL/dX_rs = sum_pq W_pq A[p][r] A[q][s] * scale   (X = tensor, W = dL/dG, G = A X A^T)
+__device__ __forceinline__ void pullback(const double A[9], const double Wre[9],
+                                         const double Wim[9], double scale, double Ore[9],
+                                         double Oim[9]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double w = A[3 * p + r] * A[3 * q + s];
+                    re += w * Wre[3 * p + q];
+                    im += w * Wim[3 * p + q];
+                }
+            Ore[3 * r + s] = re * scale;
+            Oim[3 * r + s] = im * scale;
+        }
+}
+
+// Full 3x3 dL/dG (row-major, re/im) from the gradients of the symmetric components
+// (00,11,22,01,02,12) and antisymmetric components (01,02,12) of the split.
+__device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool full,
+                                       double Wre[9], double Wim[9]) {
+    // ws/wa: interleaved re, im
+    Wre[0] = ws[0];
+    Wim[0] = ws[1];
+    Wre[4] = ws[2];
+    Wim[4] = ws[3];
+    Wre[8] = ws[4];
+    Wim[8] = ws[5];
+    const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double sr = 0.5 * ws[2 * (3 + c)], si = 0.5 * ws[2 * (3 + c) + 1];
+        const double ar = full ? 0.5 * wa[2 * c] : 0.0, ai = full ? 0.5 * wa[2 * c + 1] : 0.0;
+        Wre[3 * P[c] + Q[c]] = sr + ar;
+        Wim[3 * P[c] + Q[c]] = si + ai;
+        Wre[3 * Q[c] + P[c]] = sr - ar;
+        Wim[3 * Q[c] + P[c]] = si - ai;
+    }
+}
+
+template <bool FULL, int NB>
+__global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
+    MeshDev m, const double2* __restrict__ grad, double k0sq, int eps_kind, int mu_kind,
+    double2* __restrict__ g_eps, double2* __restrict__ g_mu) {
+    constexpr int NP = NComp<FULL>::per;
+    constexpr int NT = NComp<FULL>::total;
+    __shared__ BwdSmem<FULL, NB> sm;
+    for (int i = threadIdx.x; i < 64 * 20; i += kBlock) sm.slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
+
+    const long long tile0 = (long long)blockIdx.x * kTile;
+    const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
+    if (threadIdx.x < ntile) {
+        const int lt = threadIdx.x;
+        const long long ia = tile0 + lt;
+        const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
+        const int4 nd = m.tets[t];
+        const int n[4] = {nd.x, nd.y, nd.z, nd.w};
+        int r[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
+        const int s0 = min(n[0], n[1]), s1 = max(n[0], n[1]);
+        const int s2 = min(n[2], n[3]), s3 = max(n[2], n[3]);
+        const int w0 = min(s0, s2), t2 = max(s0, s2);
+        const int t1 = min(s1, s3), w3 = max(s1, s3);
+        const int w1 = min(t1, t2), w2 = max(t1, t2);
+        const double4 X0 = m.xyz[w0], X1 = m.xyz[w1], X2 = m.xyz[w2], X3 = m.xyz[w3];
+        const double J[9] = {X1.x - X0.x, X1.y - X0.y, X1.z - X0.z, X2.x - X0.x, X2.y - X0.y,
+                             X2.z - X0.z, X3.x - X0.x, X3.y - X0.y, X3.z - X0.z};
+        const double C[9] = {J[4] * J[8] - J[5] * J[7], J[5] * J[6] - J[3] * J[8],
+                             J[3] * J[7] - J[4] * J[6], J[7] * J[2] - J[8] * J[1],
+                             J[8] * J[0] - J[6] * J[2], J[6] * J[1] - J[7] * J[0],
+                             J[1] * J[5] - J[2] * J[4], J[2] * J[3] - J[0] * J[5],
+                             J[0] * J[4] - J[1] * J[3]};
+        const double det = J[0] * C[0] + J[1] * C[1] + J[2] * C[2];
+        const int inv = (r[0] > r[1]) + (r[0] > r[2]) + (r[0] > r[3]) + (r[1] > r[2]) +
+                        (r[1] > r[3]) + (r[2] > r[3]);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            sm.Jc[lt][i] = J[i];
+            sm.C[lt][i] = C[i];
+        }
+        sm.iad[lt] = 1.0 / fabs(det);
+        sm.sgn[lt] = ((det < 0.0) != (inv & 1)) ? -1.0 : 1.0;
+        sm.code[lt] = r[0] * 16 + r[1] * 4 + r[2];
+        sm.tet[lt] = t;
+    }
+    const int j = threadIdx.x;
+    const bool active = j < NED2_NPAIR;
+    const int pj = active ? j : 0;
+    const int a = PAIR_A[pj], b = PAIR_B[pj];
+    double tk[6], tm[6], qk[3], qm[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        tk[c] = TSYM_K[pj][c];
+        tm[c] = TSYM_M[pj][c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        qk[c] = FULL ? TANTI_K[pj][c] : 0.0;
+        qm[c] = FULL ? TANTI_M[pj][c] : 0.0;
+    }
+    const bool aodd = a >= 6, bodd = b >= 6;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+
+    for (int lt0 = 0; lt0 < ntile; lt0 += NB) {
+        const int nvalid = min(NB, ntile - lt0);
+        const double2* src = grad + (tile0 + lt0) * 400;
+        for (int i = threadIdx.x; i < nvalid * 400; i += kBlock) sm.G[i / 400][i % 400] = src[i];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            if (u < nvalid) {  // uniform across the workgroup
+                double v[NT];
+#pragma unroll
+                for (int c = 0; c < NT; ++c) v[c] = 0.0;
+                if (active) {
+                    const int lt = lt0 + u;
+                    const int code = sm.code[lt];
+                    const int ra = sm.slot[code][a], rb = sm.slot[code][b];
+                    double2 g = sm.G[u][ra * 20 + rb];
+                    if (ra != rb) {
+                        const double2 g2 = sm.G[u][rb * 20 + ra];
+                        g.x += g2.x;
+                        g.y += g2.y;
+                    }
+                    const double s = sm.sgn[lt];
+                    const double sg = (aodd ? s : 1.0) * (bodd ? s : 1.0);
+                    const double wr = sg * g.x, wi = sg * g.y;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        v[2 * c] = wr * tk[c];
+                        v[2 * c + 1] = wi * tk[c];
+                        v[NP + 2 * c] = -k0sq * wr * tm[c];
+                        v[NP + 2 * c + 1] = -k0sq * wi * tm[c];
+                    }
+                    if constexpr (FULL) {
+                        const double sa = (ra < rb) ? -1.0 : 1.0;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            v[12 + 2 * c] = sa * wr * qk[c];
+                            v[12 + 2 * c + 1] = sa * wi * qk[c];
+                            v[NP + 12 + 2 * c] = -k0sq * sa * wr * qm[c];
+                            v[NP + 12 + 2 * c + 1] = -k0sq * sa * wi * qm[c];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < NT; ++c) {
+                    const double tot = wave_sum(v[c]);
+                    if (lane == 0) sm.part[wave][u][c] = tot;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < nvalid) {
+            const int u = threadIdx.x;
+            const int lt = lt0 + u;
+            double w[NT];
+#pragma unroll
+            for (int c = 0; c < NT; ++c)
+                w[c] = sm.part[0][u][c] + sm.part[1][u][c] + sm.part[2][u][c] + sm.part[3][u][c];
+            double Wre[9], Wim[9], Ore[9], Oim[9], A[9];
+            const long long t = sm.tet[lt];
+            const double iad = sm.iad[lt];
+            if (g_mu) {
+                unsplit(w, w + 12, FULL, Wre, Wim);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) A[i] = sm.Jc[lt][i];
+                pullback(A, Wre, Wim, iad, Ore, Oim);
+                write_leaf(g_mu, t, mu_kind, Ore, Oim);
+            }
+            if (g_eps) {
+                unsplit(w + NP, w + NP + 12, FULL, Wre, Wim);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) A[i] = sm.C[lt][i];
+                pullback(A, Wre, Wim, iad, Ore, Oim);
+                write_leaf(g_eps, t, eps_kind, Ore, Oim);
+            }
+        }
+        __syncthreads();  // sm.G and sm.part are rewritten by the next sub-batch
+    }
+}
+
+}  // namespace
+
+hipError_t launch_backward(const MeshDev& m, const double2* grad, double k0sq, int eps_kind,
+                           int mu_kind, double2* g_eps, double2* g_mu, hipStream_t stream) {
+    const long long ntiles = (m.n_assy + kTile - 1) / kTile;
+    if (ntiles == 0) return hipSuccess;
+    const bool full = (g_eps && eps_kind == EM_MAT_FULL) || (g_mu && mu_kind == EM_MAT_FULL);
+    if (full)
+        hipLaunchKernelGGL((ned2_backward_kernel<true, 4>), dim3((unsigned)ntiles), dim3(kBlock), 0,
+                           stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+    else
+        hipLaunchKernelGGL((ned2_backward_kernel<false, 4>), dim3((unsigned)ntiles), dim3(kBlock),
+                           0, stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+    return hipGetLastError();
+}
+
+}  // namespace em

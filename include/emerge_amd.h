@@ -72,6 +72,7 @@ typedef struct {
 
 typedef struct em_ned2_plan_s* em_ned2_plan_t;
 typedef struct em_csc_map_s* em_csc_map_t;
+typedef struct em_graph_s* em_graph_t;
 
 const char* em_last_error(void);
 const char* em_version(void);
@@ -96,6 +97,13 @@ int em_ned2_plan_destroy(em_ned2_plan_t plan);
 /* n_assy (Ta), n_field (N), entries per COO array (400 * Ta) */
 int em_ned2_plan_sizes(em_ned2_plan_t plan, int64_t* n_assy, int64_t* n_field,
                        int64_t* nnz_coo);
+/*
+ * Launch shape of the assembly kernels of this plan: tets per LDS sub-batch (1, 2 or 4;
+ * default 2). No counterpart in the reference (Numba picks its own schedule); set by
+ * the project's autotuner (emerge_amd/autotune.py).
+ */
+int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t nb);
+int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* nb);
 
 /*
  * Element stiffness and mass blocks, reference-style (curlcurl.py:387-456):
@@ -109,12 +117,27 @@ int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er,
 /*
  * Fused A_f = K - k0[f]^2 M for n_k0 wavenumbers (assembler.py:485-494 combine step,
  * batched over a frequency sweep). out holds n_k0 COO value arrays, array f starting
- * at element f * freq_stride (freq_stride >= 400 * Ta). k0 is a HOST array.
+ * at element f * freq_stride (freq_stride >= 400 * Ta). k0 is a HOST array; its values
+ * travel as kernel arguments (one launch per 128 wavenumbers), so the call allocates
+ * nothing, never synchronises and can be captured into a HIP graph.
  * complex128 for EM_PREC_FP64, complex64 for EM_PREC_MIXED.
  */
 int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er,
                        const em_material_t* ur, const double* k0, int32_t n_k0, void* out,
                        int64_t freq_stride, int32_t precision, void* stream);
+
+/*
+ * Backward of em_ned2_assemble_a (one k0) with respect to per-tet eps_r and mu_r^-1
+ * (FP64). No counterpart in the reference, which has no gradients (its optimiser,
+ * optim.py:36-41, is derivative-free); this is the reverse of the assembly it replaces.
+ * grad_coo: dL/dA for the COO values, complex128 [400 * Ta], in PyTorch's convention
+ * dL/dRe + i dL/dIm. Outputs (either may be NULL) are complex128, tet-major, indexed by
+ * tet id: grad_eps [T * w(eps_kind)], grad_mu_inv [T * w(mu_kind)] with w = 1 (scalar:
+ * derivative w.r.t. the scalar multiplying I), 3 (diagonal) or 9 (full, row-major).
+ * Entries of tets outside the assembly are not written.
+ */
+int em_ned2_backward(em_ned2_plan_t plan, const void* grad_coo, double k0, int32_t eps_kind,
+                     void* grad_eps, int32_t mu_kind, void* grad_mu_inv, void* stream);
 
 /* The static COO pattern: rows/cols int64 [400 * Ta] (curlcurl.py:449-453). */
 int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stream);
@@ -148,6 +171,19 @@ int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_
 /* Fused to_csc(dataE - k0^2 * dataB) (assembler.py:494) without forming the COO sum. */
 int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                           void* data_csc, int32_t precision, void* stream);
+
+/*
+ * HIP graphs of assembly launches. The reference re-runs its Numba builder per call
+ * (microwave_3d.py:1457 sweep loop) and has no counterpart: these record the launches
+ * issued on `stream` between begin and end (hipStreamBeginCapture, thread-local mode)
+ * and replay them with one graph launch. Only calls that allocate nothing and never
+ * synchronise may be captured: em_ned2_assemble_km/_a, em_ned2_rowcol,
+ * em_csc_to_csc[_combine]. `stream` must not be the default stream.
+ */
+int em_graph_begin_capture(void* stream);
+int em_graph_end_capture(void* stream, em_graph_t* graph);
+int em_graph_launch(em_graph_t graph, void* stream);
+int em_graph_destroy(em_graph_t graph);
 
 #ifdef __cplusplus
 }

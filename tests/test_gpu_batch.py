@@ -1,0 +1,127 @@
+"""GPU tests of the batched launches: sub-batch variants, the autotuner, uneven
+multi-mesh batches in one launch, and frequency sweeps replayed from a HIP graph."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import ref_tensor, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+C0 = 299792458.0
+
+
+def _mats(T, rng):
+    import torch
+    eps = torch.tensor(rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T)),
+                       dtype=torch.complex128)
+    mu = torch.tensor(rng.uniform(1, 2, (T, 3)), dtype=torch.complex128)
+    return eps, mu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_subbatch_variants_identical(delaunay_field, precision):
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    plan = Ned2Plan.from_field(delaunay_field)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(1))
+    k0 = 2 * np.pi * 9e9 / C0
+    outs = []
+    for nb in (1, 2, 4):
+        plan.set_tuning(nb)
+        assert plan.get_tuning() == nb
+        outs.append(plan.assemble_a(eps, mu, k0, precision).cpu())
+        K, M = plan.assemble_km(eps, mu, precision)
+        if nb == 1:
+            K1, M1 = K.cpu(), M.cpu()
+        else:
+            assert torch.equal(K.cpu(), K1) and torch.equal(M.cpu(), M1)
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
+def test_autotuner_picks_and_caches(delaunay_field, tmp_path, monkeypatch):
+    from emerge_amd import autotune
+    from emerge_amd.assembly import Ned2Plan
+    monkeypatch.setenv("EMERGE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    plan = Ned2Plan.from_field(delaunay_field)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(2))
+    k0 = 2 * np.pi * 10e9 / C0
+    r = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, warmup=1)
+    assert r["nb"] in (1, 2, 4) and not r["cached"] and set(r["ms"]) == {1, 2, 4}
+    assert plan.get_tuning() == r["nb"]
+    plan2 = Ned2Plan.from_field(delaunay_field)
+    r2 = autotune.tune(plan2, eps, mu, k0, "mixed")
+    assert r2["cached"] and r2["nb"] == r["nb"] and plan2.get_tuning() == r["nb"]
+
+
+def test_uneven_multimesh_batch_one_launch():
+    """Meshes of uneven size in one plan: every mesh's COO slice equals the oracle's
+    assembly of that mesh alone, and its DOF map is the mesh's own plus an offset."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh, hex_box_mesh
+    fields = [Nedelec2Field(delaunay_box_mesh(n, seed=30 + i, box=(1.0, 0.5 + 0.1 * i, 2.0)))
+              for i, n in enumerate([40, 700, 90, 2500, 13])]
+    fields.insert(2, Nedelec2Field(hex_box_mesh(0.02, 0.01, 0.03, 3, 2, 4)))
+    plan = Ned2Plan.from_fields(fields)
+    T = plan.n_tets
+    rng = np.random.default_rng(3)
+    eps, mu = _mats(T, rng)
+    k0 = 2 * np.pi * 10e9 / C0
+    A = plan.assemble_a(eps, mu, k0, "fp64").cpu().numpy()
+    A32 = plan.assemble_a(eps, mu, k0, "mixed").cpu().numpy()
+    rows, cols = plan.rowcol()
+    rows, cols = rows.cpu().numpy(), cols.cpu().numpy()
+    for i, f in enumerate(fields):
+        t0, t1 = plan.tet_offsets[i], plan.tet_offsets[i + 1]
+        er = np.zeros((3, 3, f.n_tets), complex)
+        ur = np.zeros((3, 3, f.n_tets), complex)
+        e_np = eps.numpy()[t0:t1]
+        m_np = mu.numpy()[t0:t1]
+        for d in range(3):
+            er[d, d] = e_np
+            ur[d, d] = m_np[:, d]
+        dE, dB, r, c = oracle.tet_mass_stiffness_matrices(f, er, ur, [], local_frame=True)
+        sl = slice(400 * t0, 400 * t1)
+        assert rel_fro(A[sl], dE - k0 ** 2 * dB) <= 1e-12, i
+        assert rel_fro(A32[sl].astype(complex), dE - k0 ** 2 * dB) <= 1e-6, i
+        assert np.array_equal(rows[sl] - plan.dof_offsets[i], r)
+        assert np.array_equal(cols[sl] - plan.dof_offsets[i], c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sweep_graph_replays_match_eager(delaunay_field, precision):
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.graph import SweepGraph
+    plan = Ned2Plan.from_field(delaunay_field)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(4))
+    k0s = 2 * np.pi * np.linspace(8e9, 12e9, 64) / C0
+    eager = plan.assemble_a(eps, mu, k0s, precision).clone()
+    sg = SweepGraph(plan, eps, mu, k0s, precision)
+    for rep in range(3):
+        sg.out.zero_()
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        out = sg.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+        assert float(junk) == float(junk)
+    # every frequency equals the single-k0 launch
+    for f in (0, 31, 63):
+        one = plan.assemble_a(eps, mu, float(k0s[f]), precision)
+        assert torch.equal(one, eager[f])
+
+
+def test_sweep_more_than_128_frequencies(delaunay_field):
+    """Sweeps longer than one launch's wavenumber pack are split into several launches."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+    f = Nedelec2Field(delaunay_box_mesh(120, seed=7))
+    plan = Ned2Plan.from_field(f)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(5))
+    k0s = 2 * np.pi * np.linspace(1e9, 20e9, 300) / C0
+    out = plan.assemble_a(eps, mu, k0s, "mixed")
+    for fi in (0, 127, 128, 255, 256, 299):
+        assert torch.equal(out[fi], plan.assemble_a(eps, mu, float(k0s[fi]), "mixed"))

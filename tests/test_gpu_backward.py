@@ -49,7 +49,7 @@ def _grad(s, precision="fp64"):
     A = assemble_a(s["plan"], e, m, s["k0"], precision)
     L = bilinear_form(A, s["rows"], s["cols"], s["lam"], s["x"])
     L.backward()
-    return e.grad.cpu().numpy(), m.grad.cpu().numpy(), float(L)
+    return e.grad.cpu().numpy(), m.grad.cpu().numpy(), float(L.detach())
 
 
 def test_backward_matches_fp64_reference_gradient(setup):
@@ -138,6 +138,9 @@ def test_full_tensor_leaves_and_mixed_forward(setup):
         ref_m = (dK * G).reshape(T, 400).sum(dim=1)
         assert torch.linalg.norm(e.grad[:, p, q] - ref_e) <= 1e-11 * torch.linalg.norm(ref_e)
         assert torch.linalg.norm(m.grad[:, p, q] - ref_m) <= 1e-11 * torch.linalg.norm(ref_m)
+    # the mixed forward's output is complex64, so autograd hands the backward an upstream
+    # gradient rounded to complex64; the gradient itself is still computed in FP64
     ge64, gm64, _ = _grad(s, "fp64")
     ge32, gm32, _ = _grad(s, "mixed")
-    assert np.array_equal(ge64, ge32) and np.array_equal(gm64, gm32)
+    assert np.linalg.norm(ge32 - ge64) <= 1e-6 * np.linalg.norm(ge64)
+    assert np.linalg.norm(gm32 - gm64) <= 1e-6 * np.linalg.norm(gm64)

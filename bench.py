@@ -32,14 +32,14 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bytes_per_tet(out_bytes_per_entry: int, nodes_per_tet: float) -> float:
+def bytes_per_tet(out_bytes_per_entry: int, nodes_per_tet: float, mat_bytes: int = 64) -> float:
     """Algorithmic HBM bytes per tet of the fused A kernel.
 
     node ids 16 (4 x int32) + node coordinates 24 * Nn/T (3 x f64, each node read once)
-    + eps_r 16 (one complex128 per tet) + mu_r 48 (3 complex128 per tet, diagonal)
+    + materials (default: eps_r one complex128 = 16, mu_r diagonal 3 complex128 = 48)
     + 400 output entries.
     """
-    return 16 + 24 * nodes_per_tet + 16 + 48 + 400 * out_bytes_per_entry
+    return 16 + 24 * nodes_per_tet + mat_bytes + 400 * out_bytes_per_entry
 
 
 def time_launches(fn, steps: int, warmup: int):
@@ -94,6 +94,150 @@ def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int):
     return done / el, nthreads, done, el
 
 
+def _mats(T, rng, mu_kind="diag"):
+    import numpy as np
+    import torch
+    eps = torch.from_numpy(rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T)))
+    if mu_kind == "diag":
+        mu = torch.from_numpy(rng.uniform(1, 2, (T, 3)).astype(np.complex128))
+    else:
+        mu = torch.ones(T, dtype=torch.complex128)
+    return eps, mu
+
+
+def suite_sweep(args) -> dict:
+    """64 frequencies over 8-12 GHz on a ~500k-tet Delaunay mesh, eps_r = mu_r = 1, all
+    A_f in one graph launch (complex64 COO, 102 GB)."""
+    import numpy as np
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.graph import SweepGraph
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=args.seed + 3))
+    plan = Ned2Plan.from_field(f)
+    T = plan.n_tets
+    k0s = 2 * np.pi * np.linspace(8e9, 12e9, 64) / C0
+    one = plan.material(1.0)
+    sg = SweepGraph(plan, one, one, k0s, "mixed")
+    log(f"sweep: {T} tets x 64 frequencies, output {sg.out.numel() * 8 / 1e9:.1f} GB")
+    ms_graph, _ = time_launches(lambda: sg.replay(), 3, 1)
+    ms_eager, _ = time_launches(lambda: plan.assemble_a(one, one, k0s, "mixed", out=sg.out), 3, 1)
+    ref = plan.assemble_a(one, one, float(k0s[17]), "mixed")
+    sg.replay()
+    torch.cuda.synchronize()
+    same = bool(torch.equal(ref, sg.out[17]))
+    del sg, ref
+    torch.cuda.empty_cache()
+    out_bytes = 64 * T * 400 * 8
+    return {"n_tets": T, "n_freq": 64, "output": "complex64 COO per frequency",
+            "ms_per_sweep_graph": ms_graph, "ms_per_sweep_eager": ms_eager,
+            "tet_freqs_per_s": 64 * T / (ms_graph * 1e-3),
+            "write_gb_per_s": out_bytes / (ms_graph * 1e-3) / 1e9,
+            "replay_equals_single_launch": same}
+
+
+def suite_batch(args) -> dict:
+    """32 Delaunay meshes, tet counts log-uniform in [5k, 400k], per-tet complex eps_r,
+    mu_r = 1, mixed precision, one launch; per-mesh error against the FP64 kernel (which
+    matches the oracle to 1e-12 in the tests)."""
+    import numpy as np
+    import torch
+    from emerge_amd import autotune
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    rng = np.random.default_rng(args.seed + 4)
+    sizes = np.exp(rng.uniform(np.log(5e3), np.log(4e5), 32)).astype(int)
+    fields = [Nedelec2Field(delaunay_mesh_with_tets(int(s), seed=args.seed + 100 + i))
+              for i, s in enumerate(sizes)]
+    plan = Ned2Plan.from_fields(fields)
+    T = plan.n_tets
+    eps, mu = _mats(T, rng, mu_kind="one")
+    er, ur = plan.material(eps), plan.material(mu)
+    k0 = 2 * np.pi * 10e9 / C0
+    log(f"batch: 32 meshes, {T} tets ({sizes.min()}..{sizes.max()} requested)")
+    a32 = torch.empty(plan.nnz_coo, dtype=torch.complex64, device="cuda")
+    tune = autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
+    ms, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "mixed", out=a32), 10, 2)
+    a64 = plan.assemble_a(er, ur, k0, "fp64")
+    plan.assemble_a(er, ur, k0, "mixed", out=a32)
+    torch.cuda.synchronize()
+    errs = []
+    for i in range(32):
+        s0, s1 = 400 * int(plan.tet_offsets[i]), 400 * int(plan.tet_offsets[i + 1])
+        errs.append(rel_err_chunked(a32[s0:s1], a64[s0:s1]))
+    del a64, a32
+    torch.cuda.empty_cache()
+    b = bytes_per_tet(8, sum(f.mesh.n_nodes for f in fields) / T, mat_bytes=32)
+    return {"n_meshes": 32, "n_tets": T, "tets_per_mesh": [f.n_tets for f in fields],
+            "ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
+            "gb_per_s": T * b / (ms * 1e-3) / 1e9, "tuned_nb": tune["nb"],
+            "rel_err_vs_fp64_per_mesh": errs, "max_rel_err": max(errs)}
+
+
+def suite_backward(args) -> dict:
+    """dL/d eps_r and dL/d mu_r^-1 on a ~1M-tet mesh for L = Re(lam^H A x)."""
+    import numpy as np
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(1_000_000, seed=args.seed + 5))
+    plan = Ned2Plan.from_field(f)
+    T, N = plan.n_tets, f.n_field
+    rng = np.random.default_rng(args.seed + 5)
+    lam = torch.from_numpy(rng.normal(size=N) + 1j * rng.normal(size=N)).cuda()
+    x = torch.from_numpy(rng.normal(size=N) + 1j * rng.normal(size=N)).cuda()
+    rows, cols = plan.rowcol()
+    g = lam[rows] * torch.conj(x[cols])
+    del rows, cols
+    ge = torch.zeros(T, dtype=torch.complex128, device="cuda")
+    gm = torch.zeros((T, 3), dtype=torch.complex128, device="cuda")
+    from emerge_amd import _lib
+    k0 = 2 * np.pi * 10e9 / C0
+    fn = lambda: plan.backward(g, k0, _lib.EM_MAT_SCALAR, ge, _lib.EM_MAT_DIAG, gm)  # noqa: E731
+    ms, _ = time_launches(fn, 10, 2)
+    log(f"backward: {T} tets, {ms:.3f} ms")
+    b = 400 * 16 + 16 + 24 * f.mesh.n_nodes / T + 16 + 48
+    return {"n_tets": T, "ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
+            "gb_per_s": T * b / (ms * 1e-3) / 1e9, "bytes_per_tet": b,
+            "leaves": "eps_r (T,) complex, mu_r^-1 (T,3) complex"}
+
+
+def target_run(args) -> dict:
+    """The large single-mesh case: mixed and FP64 fused A on a ~8M-tet Delaunay mesh."""
+    import numpy as np
+    import torch
+    from emerge_amd import autotune
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+    f = Nedelec2Field(delaunay_box_mesh(args.target_points, seed=args.seed + 6))
+    plan = Ned2Plan.from_field(f)
+    T, Nn = plan.n_tets, f.mesh.n_nodes
+    eps, mu = _mats(T, np.random.default_rng(args.seed + 6))
+    er, ur = plan.material(eps), plan.material(mu)
+    k0 = 2 * np.pi * 10e9 / C0
+    log(f"target: {T} tets")
+    res = {"n_tets": T, "points": args.target_points}
+    a32 = torch.empty(plan.nnz_coo, dtype=torch.complex64, device="cuda")
+    autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
+    ms32, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "mixed", out=a32), 10, 2)
+    a64 = torch.empty(plan.nnz_coo, dtype=torch.complex128, device="cuda")
+    autotune.tune(plan, er, ur, k0, "fp64", out=a64[None], reps=3)
+    ms64, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "fp64", out=a64), 10, 2)
+    autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
+    plan.assemble_a(er, ur, k0, "mixed", out=a32)
+    torch.cuda.synchronize()
+    err = rel_err_chunked(a32, a64)
+    for name, ms, c in (("mixed", ms32, 8), ("fp64", ms64, 16)):
+        b = bytes_per_tet(c, Nn / T)
+        res[name] = {"ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
+                     "gb_per_s": T * b / (ms * 1e-3) / 1e9,
+                     "roofline_frac": T * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    res["rel_err_mixed_vs_fp64"] = err
+    del a32, a64
+    torch.cuda.empty_cache()
+    return res
+
+
 def load_traffic(workload_key: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and
     calibrated on a copy kernel; committed under profiles/ by the profiling recipe."""
@@ -117,6 +261,12 @@ def main() -> int:
     ap.add_argument("--cpu-chunk", type=int, default=50_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--suite", action="store_true",
+                    help="also measure the frequency sweep (500k tets x 64 f), the uneven "
+                         "32-mesh batch and the backward on 1M tets")
+    ap.add_argument("--target-points", type=int, default=0,
+                    help="also run the large single-mesh case with this many points "
+                         "(1200000 gives about 8M tets)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 2 GiB device copy, so that a rocprofv3 --pmc "
                          "run can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -152,7 +302,9 @@ def main() -> int:
     nnz = plan.nnz_coo
     log(f"plan ready ({time.perf_counter() - t0:.1f}s); COO entries {nnz}")
 
+    from emerge_amd import autotune
     a32 = torch.empty(nnz, dtype=torch.complex64, device="cuda")
+    tuning = {"mixed": autotune.tune(plan, eps, mu, k0, "mixed", out=a32[None])["nb"]}
     ms32, wall32 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32),
                                  args.steps, args.warmup)
     log(f"mixed: {ms32:.3f} ms/launch")
@@ -160,9 +312,11 @@ def main() -> int:
     err = None
     if not args.no_fp64:
         a64 = torch.empty(nnz, dtype=torch.complex128, device="cuda")
+        tuning["fp64"] = autotune.tune(plan, eps, mu, k0, "fp64", out=a64[None])["nb"]
         ms64, wall64 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "fp64", out=a64),
                                      args.steps, args.warmup)
         log(f"fp64: {ms64:.3f} ms/launch")
+        plan.set_tuning(tuning["mixed"])
         plan.assemble_a(eps, mu, k0, "mixed", out=a32)
         torch.cuda.synchronize()
         err = rel_err_chunked(a32, a64)
@@ -228,8 +382,24 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_tet": b32, "kernel": "ned2_assemble_kernel<float, MODE_A>"},
         "fp64": res64,
+        "tuning_nb": tuning,
         "cpu_baseline": cpu,
     }
+    if args.suite or args.target_points:
+        del a32
+        plan.close()
+        torch.cuda.empty_cache()
+        extra = {}
+        if args.suite:
+            log("suite: frequency sweep")
+            extra["sweep_500k_x64"] = suite_sweep(args)
+            log("suite: uneven 32-mesh batch")
+            extra["batch_32_meshes"] = suite_batch(args)
+            log("suite: backward")
+            extra["backward_1M"] = suite_backward(args)
+        if args.target_points:
+            extra["target"] = target_run(args)
+        out["configs"] = extra
     print(json.dumps(out), flush=True)
     return 0
 

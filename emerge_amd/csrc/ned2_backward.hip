@@ -10,8 +10,10 @@
 //         - k0^2 (sum_c gm_c TSYM_M[p][c] + sa sum_c am_c TANTI_M[p][c]))
 // for its canonical pair p, so with w = sg * (g(ii,jj) + g(jj,ii)) (g(ii,ii) once)
 //   dL/dgk_c = sum_p w TSYM_K[p][c],  dL/dgm_c = -k0^2 sum_p w TSYM_M[p][c]
-// (and the antisymmetric parts with sa). Those sums are wavefront shuffle reductions
-// over the 210 pair-threads. Then, from G_M = C eps C^T / |det J| and
+// (and the antisymmetric parts with sa). Per round of 8 tets the 210 pair-threads write
+// the signed weights w into LDS (reading the upstream block straight from HBM, each
+// entry once), then 192 threads form the 8 x 12 dot products of length 210 with the
+// reference tensors held in LDS (a small GEMM). Then, from G_M = C eps C^T / |det J| and
 // G_K = Jc mu^-1 Jc^T / |det J| (Jc rows = columns of J):
 //   dL/deps = C^T W_M C / |det J|,   dL/dmu^-1 = Jc^T W_K Jc / |det J|
 // reduced to the leaf's kind (scalar: trace, diagonal: diagonal, full: all 9).
@@ -29,31 +31,29 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTile = 64;
 
+// Work split of one round of NW tets: the dot-product phase runs NW x NC x NSPLIT threads.
 template <bool FULL>
-struct NComp {
-    // per matrix: 6 symmetric (+3 antisymmetric) complex components, as doubles
-    static constexpr int per = FULL ? 18 : 12;
-    static constexpr int total = 2 * per;
+struct Shape {
+    static constexpr int NW = 8;                  // tets per round
+    static constexpr int NC = FULL ? 18 : 12;     // complex outputs per tet (K then M)
+    static constexpr int NSPLIT = FULL ? 1 : 2;   // pair ranges per output
 };
 
-template <bool FULL, int NB>
+template <bool FULL>
 struct BwdSmem {
+    static constexpr int NW = Shape<FULL>::NW, NC = Shape<FULL>::NC, NS = Shape<FULL>::NSPLIT;
     double Jc[kTile][9];
     double C[kTile][9];
     double iad[kTile];
     double sgn[kTile];
     long long tet[kTile];
     int code[kTile];
-    alignas(16) double2 G[NB][400];
-    double part[4][NB][NComp<FULL>::total];
+    double T[NED2_NPAIR][NC];          // reference tensors, M part scaled by -k0^2
+    double2 W[NW][NED2_NPAIR];         // signed pair weights
+    double2 Wa[FULL ? NW : 1][NED2_NPAIR];  // times the antisymmetric sign
+    double2 R[NS][NW][NC];
     unsigned char slot[64][20];
 };
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // g_out (tet-major, complex) for a 3x3 complex gradient Gr (row-major) reduced to kind.
 __device__ __forceinline__ void write_leaf(double2* out, long long t, int kind,
@@ -115,14 +115,26 @@ __device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool
     }
 }
 
-template <bool FULL, int NB>
+template <bool FULL>
 __global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
     MeshDev m, const double2* __restrict__ grad, double k0sq, int eps_kind, int mu_kind,
     double2* __restrict__ g_eps, double2* __restrict__ g_mu) {
-    constexpr int NP = NComp<FULL>::per;
-    constexpr int NT = NComp<FULL>::total;
-    __shared__ BwdSmem<FULL, NB> sm;
+    using S = Shape<FULL>;
+    constexpr int NW = S::NW, NC = S::NC, NS = S::NSPLIT;
+    constexpr int NP = NC;  // doubles per matrix in the flattened (re, im) result
+    __shared__ BwdSmem<FULL> sm;
     for (int i = threadIdx.x; i < 64 * 20; i += kBlock) sm.slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
+    for (int i = threadIdx.x; i < NED2_NPAIR * NC; i += kBlock) {
+        const int p = i / NC, c = i % NC;
+        double v;
+        if (FULL) {
+            v = c < 6 ? TSYM_K[p][c] : c < 9 ? TANTI_K[p][c - 6]
+                : c < 15 ? -k0sq * TSYM_M[p][c - 9] : -k0sq * TANTI_M[p][c - 15];
+        } else {
+            v = c < 6 ? TSYM_K[p][c] : -k0sq * TSYM_M[p][c - 6];
+        }
+        sm.T[p][c] = v;
+    }
 
     const long long tile0 = (long long)blockIdx.x * kTile;
     const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
@@ -163,81 +175,73 @@ __global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
         sm.tet[lt] = t;
     }
     const int j = threadIdx.x;
-    const bool active = j < NED2_NPAIR;
-    const int pj = active ? j : 0;
+    const bool pair_thread = j < NED2_NPAIR;
+    const int pj = pair_thread ? j : 0;
     const int a = PAIR_A[pj], b = PAIR_B[pj];
-    double tk[6], tm[6], qk[3], qm[3];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        tk[c] = TSYM_K[pj][c];
-        tm[c] = TSYM_M[pj][c];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        qk[c] = FULL ? TANTI_K[pj][c] : 0.0;
-        qm[c] = FULL ? TANTI_M[pj][c] : 0.0;
-    }
     const bool aodd = a >= 6, bodd = b >= 6;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // dot-phase role: (u, c, split)
+    const bool dot_thread = threadIdx.x < NW * NC * NS;
+    const int du = threadIdx.x / (NC * NS);
+    const int dc = (threadIdx.x / NS) % NC;
+    const int dsp = threadIdx.x % NS;
+    const bool anti_c = FULL && ((dc >= 6 && dc < 9) || dc >= 15);
+    const int p_lo = dsp * ((NED2_NPAIR + NS - 1) / NS);
+    const int p_hi = min(NED2_NPAIR, p_lo + (NED2_NPAIR + NS - 1) / NS);
     __syncthreads();
 
-    for (int lt0 = 0; lt0 < ntile; lt0 += NB) {
-        const int nvalid = min(NB, ntile - lt0);
-        const double2* src = grad + (tile0 + lt0) * 400;
-        for (int i = threadIdx.x; i < nvalid * 400; i += kBlock) sm.G[i / 400][i % 400] = src[i];
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            if (u < nvalid) {  // uniform across the workgroup
-                double v[NT];
-#pragma unroll
-                for (int c = 0; c < NT; ++c) v[c] = 0.0;
-                if (active) {
-                    const int lt = lt0 + u;
-                    const int code = sm.code[lt];
-                    const int ra = sm.slot[code][a], rb = sm.slot[code][b];
-                    double2 g = sm.G[u][ra * 20 + rb];
-                    if (ra != rb) {
-                        const double2 g2 = sm.G[u][rb * 20 + ra];
-                        g.x += g2.x;
-                        g.y += g2.y;
-                    }
-                    const double s = sm.sgn[lt];
-                    const double sg = (aodd ? s : 1.0) * (bodd ? s : 1.0);
-                    const double wr = sg * g.x, wi = sg * g.y;
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        v[2 * c] = wr * tk[c];
-                        v[2 * c + 1] = wi * tk[c];
-                        v[NP + 2 * c] = -k0sq * wr * tm[c];
-                        v[NP + 2 * c + 1] = -k0sq * wi * tm[c];
-                    }
-                    if constexpr (FULL) {
-                        const double sa = (ra < rb) ? -1.0 : 1.0;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            v[12 + 2 * c] = sa * wr * qk[c];
-                            v[12 + 2 * c + 1] = sa * wi * qk[c];
-                            v[NP + 12 + 2 * c] = -k0sq * sa * wr * qm[c];
-                            v[NP + 12 + 2 * c + 1] = -k0sq * sa * wi * qm[c];
-                        }
-                    }
+    for (int lt0 = 0; lt0 < ntile; lt0 += NW) {
+        const int nvalid = min(NW, ntile - lt0);
+        // pair phase: signed weight of every canonical pair of every tet of the round
+        if (pair_thread) {
+            for (int u = 0; u < nvalid; ++u) {
+                const int lt = lt0 + u;
+                const int code = sm.code[lt];
+                const int ra = sm.slot[code][a], rb = sm.slot[code][b];
+                const double2* blk = grad + (tile0 + lt) * 400;
+                double2 g = blk[ra * 20 + rb];
+                if (ra != rb) {
+                    const double2 g2 = blk[rb * 20 + ra];
+                    g.x += g2.x;
+                    g.y += g2.y;
                 }
-#pragma unroll
-                for (int c = 0; c < NT; ++c) {
-                    const double tot = wave_sum(v[c]);
-                    if (lane == 0) sm.part[wave][u][c] = tot;
+                const double s = sm.sgn[lt];
+                const double sg = (aodd ? s : 1.0) * (bodd ? s : 1.0);
+                sm.W[u][j] = make_double2(sg * g.x, sg * g.y);
+                if constexpr (FULL) {
+                    const double sa = (ra < rb) ? -sg : sg;
+                    sm.Wa[u][j] = make_double2(sa * g.x, sa * g.y);
                 }
             }
+        }
+        __syncthreads();
+        // dot phase: R[split][u][c] = sum over pairs of W[u][p] * T[p][c]
+        if (dot_thread && du < nvalid) {
+            const double2* w = (FULL && anti_c) ? sm.Wa[FULL ? du : 0] : sm.W[du];
+            double re = 0.0, im = 0.0;
+            for (int p = p_lo; p < p_hi; ++p) {
+                const double2 v = w[p];
+                const double tv = sm.T[p][dc];
+                re += v.x * tv;
+                im += v.y * tv;
+            }
+            sm.R[dsp][du][dc] = make_double2(re, im);
         }
         __syncthreads();
         if (threadIdx.x < nvalid) {
             const int u = threadIdx.x;
             const int lt = lt0 + u;
-            double w[NT];
+            double w[2 * NC];
 #pragma unroll
-            for (int c = 0; c < NT; ++c)
-                w[c] = sm.part[0][u][c] + sm.part[1][u][c] + sm.part[2][u][c] + sm.part[3][u][c];
+            for (int c = 0; c < NC; ++c) {
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    re += sm.R[q][u][c].x;
+                    im += sm.R[q][u][c].y;
+                }
+                w[2 * c] = re;
+                w[2 * c + 1] = im;
+            }
             double Wre[9], Wim[9], Ore[9], Oim[9], A[9];
             const long long t = sm.tet[lt];
             const double iad = sm.iad[lt];
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
                 write_leaf(g_eps, t, eps_kind, Ore, Oim);
             }
         }
-        __syncthreads();  // sm.G and sm.part are rewritten by the next sub-batch
+        __syncthreads();  // the next round's dot phase rewrites R
     }
 }
 
@@ -268,11 +272,11 @@ hipError_t launch_backward(const MeshDev& m, const double2* grad, double k0sq, i
     if (ntiles == 0) return hipSuccess;
     const bool full = (g_eps && eps_kind == EM_MAT_FULL) || (g_mu && mu_kind == EM_MAT_FULL);
     if (full)
-        hipLaunchKernelGGL((ned2_backward_kernel<true, 4>), dim3((unsigned)ntiles), dim3(kBlock), 0,
+        hipLaunchKernelGGL((ned2_backward_kernel<true>), dim3((unsigned)ntiles), dim3(kBlock), 0,
                            stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
     else
-        hipLaunchKernelGGL((ned2_backward_kernel<false, 4>), dim3((unsigned)ntiles), dim3(kBlock),
-                           0, stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+        hipLaunchKernelGGL((ned2_backward_kernel<false>), dim3((unsigned)ntiles), dim3(kBlock), 0,
+                           stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
     return hipGetLastError();
 }
 

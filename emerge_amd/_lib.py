@@ -8,7 +8,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libemerge_amd.so")
+# EMERGE_AMD_LIB points at another build of the same C-ABI (e.g. one built with different
+# compile-time options); the default is the in-tree library.
+LIB_PATH = os.environ.get("EMERGE_AMD_LIB", os.path.join(_HERE, "libemerge_amd.so"))
 
 EM_OK = 0
 EM_EINVAL = -1

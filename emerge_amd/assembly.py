@@ -194,9 +194,9 @@ class Ned2Plan:
         plan.dof_offsets = np.asarray(doffs, np.int64)
         return plan
 
-    def set_tuning(self, nb: int) -> None:
-        """Tets per LDS sub-batch of the assembly kernels (1, 2 or 4)."""
-        check(_lib.lib().em_ned2_plan_set_tuning(self._h, int(nb)))
+    def set_tuning(self, tile: int) -> None:
+        """Tets per workgroup tile of the assembly kernels (32, 64 or 128)."""
+        check(_lib.lib().em_ned2_plan_set_tuning(self._h, int(tile)))
 
     def get_tuning(self) -> int:
         v = ctypes.c_int32()

@@ -1,8 +1,9 @@
 """Autotuner for the assembly kernels' launch shape.
 
-The tunable is the number of tets per LDS sub-batch (1, 2 or 4): it trades LDS per
-workgroup (occupancy) against barriers per tet, and the best value depends on the
-precision, the output mode (one k0 or a sweep), the material kinds and the mesh size.
+The tunable is the number of tets per workgroup tile (32, 64 or 128): it trades the
+latency of the per-tile geometry phase and the tail of partly filled tiles against
+occupancy, and the best value depends on the precision, the output mode (one k0 or a
+sweep), the material kinds and the mesh size.
 ``tune`` times each candidate with HIP events on the launch stream, applies the fastest
 to the plan and remembers it in a JSON cache keyed by all of the above, so later plans
 of the same kind skip the timing.
@@ -18,7 +19,7 @@ import os
 import numpy as np
 import torch
 
-CANDIDATES = (1, 2, 4)
+CANDIDATES = (32, 64, 128)
 
 
 def cache_path() -> str:
@@ -71,7 +72,7 @@ def _time(fn, reps: int, warmup: int) -> float:
 
 def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, warmup: int = 2,
          use_cache: bool = True, candidates=CANDIDATES) -> dict:
-    """Pick the fastest sub-batch size for ``plan.assemble_a(er, ur, k0, precision)``.
+    """Pick the fastest tile size for ``plan.assemble_a(er, ur, k0, precision)``.
 
     Returns ``{"nb": best, "ms": {nb: ms}, "cached": bool, "key": key}`` and leaves the
     plan set to the best value.

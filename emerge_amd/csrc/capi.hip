@@ -6,6 +6,7 @@
 // (src/emerge/_emerge/physics/microwave/assembly/curlcurl.py:387-417).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +31,18 @@ struct em_ned2_plan_s {
     int4* tets = nullptr;
     int* t2f = nullptr;
     int* tetids = nullptr;
-    int nb = 2;  // tets per LDS sub-batch of the assembly kernel (autotuned)
+    int tile = 64;  // tets per workgroup tile of the assembly kernel (autotuned)
+    int* order = nullptr;           // assembly indices grouped by vertex-rank code
+    int2* tiles[3] = {nullptr, nullptr, nullptr};
+    long long ntiles[3] = {0, 0, 0};
+
+    void tilesets(em::TileSet ts[3]) const {
+        for (int i = 0; i < 3; ++i) {
+            ts[i].order = order;
+            ts[i].tiles = tiles[i];
+            ts[i].ntiles = ntiles[i];
+        }
+    }
 
     em::MeshDev dev() const {
         em::MeshDev m;
@@ -84,6 +96,8 @@ void free_plan(em_ned2_plan_s* p) {
     (void)hipFree(p->tets);
     (void)hipFree(p->t2f);
     (void)hipFree(p->tetids);
+    (void)hipFree(p->order);
+    for (int i = 0; i < 3; ++i) (void)hipFree(p->tiles[i]);
     delete p;
 }
 
@@ -203,6 +217,35 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         }
     }
 
+    // Group the assembly indices by vertex-rank code (stable counting sort) and cut each
+    // group into tiles of 32, 64 and 128 tets.
+    std::vector<int> horder((size_t)n_assy);
+    std::vector<std::vector<int2>> htiles(3);
+    {
+        std::vector<unsigned char> code((size_t)n_assy);
+        long long cnt[65] = {0};
+        for (long long ia = 0; ia < n_assy; ++ia) {
+            const int4 v = htets[tetids ? hids[ia] : ia];
+            const int n[4] = {v.x, v.y, v.z, v.w};
+            int r[4];
+            for (int a = 0; a < 4; ++a)
+                r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
+            code[ia] = (unsigned char)(r[0] * 16 + r[1] * 4 + r[2]);
+            cnt[code[ia] + 1]++;
+        }
+        for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
+        std::vector<long long> cur(cnt, cnt + 64);
+        for (long long ia = 0; ia < n_assy; ++ia) horder[cur[code[ia]]++] = (int)ia;
+        for (int k = 0; k < 3; ++k) {
+            const int ts = em::kTileSizes[k];
+            for (int c = 0; c < 64; ++c)
+                for (long long s0 = cnt[c]; s0 < cnt[c + 1]; s0 += ts) {
+                    const long long n = std::min<long long>(ts, cnt[c + 1] - s0);
+                    htiles[k].push_back(make_int2((int)s0, (int)(n | ((long long)c << 16))));
+                }
+        }
+    }
+
     DeviceGuard g(device);
     if (!g.ok) {
         set_error("plan_create: cannot select device");
@@ -216,10 +259,14 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
     p->n_field = n_field;
     int st;
     if ((st = dmalloc(&p->xyz, hxyz.size())) || (st = dmalloc(&p->tets, htets.size())) ||
-        (st = dmalloc(&p->t2f, ht2f.size())) || (st = dmalloc(&p->tetids, hids.size()))) {
+        (st = dmalloc(&p->t2f, ht2f.size())) || (st = dmalloc(&p->tetids, hids.size())) ||
+        (st = dmalloc(&p->order, horder.size())) || (st = dmalloc(&p->tiles[0], htiles[0].size())) ||
+        (st = dmalloc(&p->tiles[1], htiles[1].size())) ||
+        (st = dmalloc(&p->tiles[2], htiles[2].size()))) {
         free_plan(p);
         return st;
     }
+    for (int k = 0; k < 3; ++k) p->ntiles[k] = (long long)htiles[k].size();
     auto cp = [&](void* d, const void* h, size_t bytes) -> int {
         if (bytes == 0) return EM_OK;
         hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
@@ -232,7 +279,11 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
     if ((st = cp(p->xyz, hxyz.data(), hxyz.size() * sizeof(double4))) ||
         (st = cp(p->tets, htets.data(), htets.size() * sizeof(int4))) ||
         (st = cp(p->t2f, ht2f.data(), ht2f.size() * sizeof(int))) ||
-        (st = cp(p->tetids, hids.data(), hids.size() * sizeof(int)))) {
+        (st = cp(p->tetids, hids.data(), hids.size() * sizeof(int))) ||
+        (st = cp(p->order, horder.data(), horder.size() * sizeof(int))) ||
+        (st = cp(p->tiles[0], htiles[0].data(), htiles[0].size() * sizeof(int2))) ||
+        (st = cp(p->tiles[1], htiles[1].data(), htiles[1].size() * sizeof(int2))) ||
+        (st = cp(p->tiles[2], htiles[2].data(), htiles[2].size() * sizeof(int2)))) {
         free_plan(p);
         return st;
     }
@@ -328,9 +379,11 @@ int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er, const em_m
         return EM_EINVAL;
     }
     DeviceGuard g(plan->device);
-    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true), em::MODE_KM,
-                               precision, needs_anti(er, ur), nullptr, 0, dataE, dataB, 0,
-                               plan->nb, (hipStream_t)stream));
+    em::TileSet ts[3];
+    plan->tilesets(ts);
+    EM_HIP(em::launch_assemble(plan->dev(), ts, plan->tile, to_desc(er, false), to_desc(ur, true),
+                               em::MODE_KM, precision, needs_anti(er, ur), nullptr, 0, dataE,
+                               dataB, 0, (hipStream_t)stream));
     return EM_OK;
 }
 
@@ -373,10 +426,12 @@ int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er, const em_ma
         k0sq[f] = k0[f] * k0[f];
     }
     DeviceGuard g(plan->device);
-    EM_HIP(em::launch_assemble(plan->dev(), to_desc(er, false), to_desc(ur, true),
+    em::TileSet ts[3];
+    plan->tilesets(ts);
+    EM_HIP(em::launch_assemble(plan->dev(), ts, plan->tile, to_desc(er, false), to_desc(ur, true),
                                n_k0 == 1 ? em::MODE_A : em::MODE_AF, precision,
                                needs_anti(er, ur), k0sq.data(), n_k0, out, nullptr, freq_stride,
-                               plan->nb, (hipStream_t)stream));
+                               (hipStream_t)stream));
     return EM_OK;
 }
 
@@ -455,25 +510,25 @@ int em_graph_destroy(em_graph_t graph) {
     return EM_OK;
 }
 
-int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t nb) {
+int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t tile) {
     if (!plan) {
         set_error("null plan");
         return EM_EINVAL;
     }
-    if (nb != 1 && nb != 2 && nb != 4) {
-        set_error("tuning: tets per sub-batch must be 1, 2 or 4");
+    if (tile != 32 && tile != 64 && tile != 128) {
+        set_error("tuning: tets per tile must be 32, 64 or 128");
         return EM_EINVAL;
     }
-    plan->nb = nb;
+    plan->tile = tile;
     return EM_OK;
 }
 
-int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* nb) {
-    if (!plan || !nb) {
+int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* tile) {
+    if (!plan || !tile) {
         set_error("null argument");
         return EM_EINVAL;
     }
-    *nb = plan->nb;
+    *tile = plan->tile;
     return EM_OK;
 }
 

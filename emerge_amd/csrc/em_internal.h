@@ -37,13 +37,24 @@ struct K0Pack {
     double v[kMaxK0];  // k0^2
 };
 
+// Tets grouped by vertex-rank code: order[] lists assembly indices bucket by bucket
+// (ascending within a bucket), tiles[] = (first position in order, count | code << 16)
+// with at most `tile` tets each.
+struct TileSet {
+    const int* order = nullptr;
+    const int2* tiles = nullptr;
+    long long ntiles = 0;
+};
+constexpr int kTileSizes[3] = {32, 64, 128};
+
 // Launchers (defined in ned2_assemble.hip). All return hipError_t of the launch.
+// ts: the three TileSets for tile sizes 32, 64, 128; tile selects one.
 // k0sq: host array of nf values of k0^2 (MODE_A uses k0sq[0]; MODE_KM ignores it).
-// nb: tets per LDS sub-batch (1, 2 or 4; other values mean 2).
-hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
-                           int mode, int precision, int anti, const double* k0sq, int nf,
-                           void* out0, void* out1, long long freq_stride, int nb,
-                           hipStream_t stream);
+// full: material tensors may be non-diagonal (general 3x3 path with antisymmetric terms).
+hipError_t launch_assemble(const MeshDev& m, const TileSet* ts, int tile, const MatDesc& eps,
+                           const MatDesc& mu, int mode, int precision, int full,
+                           const double* k0sq, int nf, void* out0, void* out1,
+                           long long freq_stride, hipStream_t stream);
 hipError_t launch_rowcol(const MeshDev& m, long long* rows, long long* cols, hipStream_t s);
 hipError_t launch_check_tables(const int4* tets, long long T, const long long* edges,
                                long long E, const long long* tris, long long F,

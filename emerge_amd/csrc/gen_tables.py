@@ -192,6 +192,23 @@ def render() -> str:
         lines.append("  {" + ", ".join(str(v) for v in row) + "},")
     lines.append("};")
     lines.append("static constexpr signed char PERM_PARITY[64] = {" + ", ".join(str(p) for p in parity) + "};")
+    # CANON[code][slot]: canonical function held by reference slot (inverse of SLOT_OF)
+    canon = [[0] * 20 for _ in range(64)]
+    for code in range(64):
+        if parity[code] == 0:
+            continue
+        for c, slot in enumerate(slot_of[code]):
+            canon[code][slot] = c
+    lines.append("static constexpr unsigned char CANON[64][20] = {")
+    for row in canon:
+        lines.append("  {" + ", ".join(str(v) for v in row) + "},")
+    lines.append("};")
+    # PAIR_INDEX[a][b]: packed index of the unordered pair {a, b}
+    index = {ab: n for n, ab in enumerate(pr)}
+    lines.append("static constexpr unsigned char PAIR_INDEX[20][20] = {")
+    for a in range(20):
+        lines.append("  {" + ", ".join(str(index[(min(a, b), max(a, b))]) for b in range(20)) + "},")
+    lines.append("};")
     return "\n".join(lines) + "\n"
 
 

@@ -138,18 +138,6 @@ __device__ __forceinline__ void congruence(const double A[3][3], const Cd E[9], 
         }
 }
 
-// LDS record of one tet: symmetric (6) parts of G_K and G_M, their antisymmetric (3)
-// parts when FULL, the rank code and the orientation sign.
-template <typename R, bool FULL>
-struct TileRec {
-    R gk[kTile][6][2];
-    R gm[kTile][6][2];
-    R ak[FULL ? kTile : 1][3][2];
-    R am[FULL ? kTile : 1][3][2];
-    int code[kTile];
-    R sgn[kTile];
-};
-
 template <typename R>
 __device__ __forceinline__ void split_store(const Cd G[9], R (*sym)[2], R (*anti)[2]) {
     const Cd s01 = cscale(cadd(G[1], G[3]), 0.5), s02 = cscale(cadd(G[2], G[6]), 0.5),
@@ -204,12 +192,14 @@ __device__ __forceinline__ void diag_congruence_store(const double A[3][3], cons
     }
 }
 
-// Phase 1 for one tet (FP64 throughout).
+// Metric tensors of one tet (FP64 throughout): writes the symmetric (and, when FULL,
+// antisymmetric) parts of G_K and of mscale * G_M, and returns the vertex-rank code and
+// the orientation sign.
 template <typename R, bool FULL>
 __device__ __forceinline__ void tet_geometry(const MeshDev& m, const MatDesc& eps,
-                                             const MatDesc& mu, long long ia, int lt,
-                                             TileRec<R, FULL>& rec) {
-    const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
+                                             const MatDesc& mu, long long t, double mscale,
+                                             R (*gk)[2], R (*gm)[2], R (*ak)[2], R (*am)[2],
+                                             int& code, R& sgn) {
     const int4 nd = m.tets[t];
     const int n[4] = {nd.x, nd.y, nd.z, nd.w};
     int r[4];
@@ -238,40 +228,33 @@ __device__ __forceinline__ void tet_geometry(const MeshDev& m, const MatDesc& ep
          Jc[0][0] * Jc[1][1] - Jc[0][1] * Jc[1][0]}};
     const double det = Jc[0][0] * C[0][0] + Jc[0][1] * C[0][1] + Jc[0][2] * C[0][2];
     const double inv_adet = 1.0 / fabs(det);
-    const int code = r[0] * 16 + r[1] * 4 + r[2];
+    code = r[0] * 16 + r[1] * 4 + r[2];
     const int inv = (r[0] > r[1]) + (r[0] > r[2]) + (r[0] > r[3]) + (r[1] > r[2]) +
                     (r[1] > r[3]) + (r[2] > r[3]);
-    const double s = ((det < 0.0) != (inv & 1)) ? -1.0 : 1.0;
+    sgn = ((det < 0.0) != (inv & 1)) ? (R)-1 : (R)1;
 
     if constexpr (!FULL) {
         Cd e[3];
         load_diag(eps, t, e);
-        diag_congruence_store<R>(C, e, inv_adet, rec.gm[lt]);
+        diag_congruence_store<R>(C, e, inv_adet * mscale, gm);
         load_diag(mu, t, e);
         if (mu.invert) {
             const Cd one = {1.0, 0.0};
 #pragma unroll
             for (int d = 0; d < 3; ++d) e[d] = cdiv(one, e[d]);
         }
-        diag_congruence_store<R>(Jc, e, inv_adet, rec.gk[lt]);
+        diag_congruence_store<R>(Jc, e, inv_adet, gk);
     } else {
         Cd E[9], G[9];
         load_tensor(eps, t, E);
-        congruence(C, E, eps.kind, inv_adet, G);
-        split_store<R>(G, rec.gm[lt], rec.am[lt]);
+        congruence(C, E, eps.kind, inv_adet * mscale, G);
+        split_store<R>(G, gm, am);
         load_tensor(mu, t, E);
         if (mu.invert) invert_tensor(E);
         congruence(Jc, E, mu.kind, inv_adet, G);
-        split_store<R>(G, rec.gk[lt], rec.ak[lt]);
+        split_store<R>(G, gk, ak);
     }
-    rec.code[lt] = code;
-    rec.sgn[lt] = (R)s;
 }
-
-template <typename R>
-struct alignas(2 * sizeof(R)) C2 {
-    R re, im;
-};
 
 template <typename R>
 struct OutVec;
@@ -284,168 +267,166 @@ struct OutVec<float> {
     using T = float2;
 };
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-// Copy `bytes` (multiple of 16) from LDS to global with 16-byte streaming stores.
-__device__ __forceinline__ void stream_out(const void* lds, void* gbl, int bytes) {
-    const u32x4* s = reinterpret_cast<const u32x4*>(lds);
-    u32x4* d = reinterpret_cast<u32x4*>(gbl);
-    const int n = bytes >> 4;
-    for (int i = threadIdx.x; i < n; i += kBlock) __builtin_nontemporal_store(s[i], d + i);
-}
-
-template <typename R, int MODE, bool ANTI, int NB>
-struct Smem {
-    static constexpr int kStageMats = (MODE == MODE_A) ? 1 : 2;
-    TileRec<R, ANTI> rec;
-    alignas(16) C2<R> stage[2][kStageMats][NB * 400];
-    unsigned char slot[64][20];
+// LDS record of a tile's tets: metric tensors, orientation sign, assembly index.
+template <typename R, bool FULL, int TILE>
+struct StreamRec {
+    R gk[TILE][6][2];
+    R gm[TILE][6][2];
+    R ak[FULL ? TILE : 1][3][2];
+    R am[FULL ? TILE : 1][3][2];
+    R sgn[TILE];
+    int ia[TILE];
 };
 
-template <typename R, int MODE, bool ANTI, int NB>
+// Store a thread's two entries of one block with 16-byte streaming stores: complex64
+// entries e0 and e1 = e0 + 1 together, complex128 entries e0 and e1 separately.
+__device__ __forceinline__ void store2(float2* blk, int e0, int e1, float v0r, float v0i,
+                                       float v1r, float v1i) {
+    (void)e1;
+    const f32x4 v = {v0r, v0i, v1r, v1i};
+    __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(blk + e0));
+}
+__device__ __forceinline__ void store2(double2* blk, int e0, int e1, double v0r, double v0i,
+                                       double v1r, double v1i) {
+    const f64x2 a = {v0r, v0i}, b = {v1r, v1i};
+    __builtin_nontemporal_store(a, reinterpret_cast<f64x2*>(blk + e0));
+    __builtin_nontemporal_store(b, reinterpret_cast<f64x2*>(blk + e1));
+}
+
+// One workgroup per tile: up to TILE tets that share one vertex-rank code (so every
+// output slot maps to the same canonical pair for all of them). Threads 0..199 own two
+// output entries of every tet's 20x20 block, with the reference-tensor coefficients of
+// their two canonical pairs in registers; they read each tet's metric tensors from LDS
+// (broadcast) and stream their 16 bytes (complex64) or 32 bytes (complex128) straight to
+// the COO arrays: no LDS staging, one barrier per tile.
+template <typename R, int MODE, bool FULL, int TILE>
 __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
-    MeshDev m, MatDesc eps, MatDesc mu, K0Pack k0p, int nf, void* __restrict__ out0,
-    void* __restrict__ out1, long long freq_stride) {
+    MeshDev m, MatDesc eps, MatDesc mu, K0Pack k0p, int nf, const int* __restrict__ order,
+    const int2* __restrict__ tiles, void* __restrict__ out0, void* __restrict__ out1,
+    long long freq_stride) {
     using O = typename OutVec<R>::T;
-    __shared__ Smem<R, MODE, ANTI, NB> sm;
-    constexpr int NS = Smem<R, MODE, ANTI, NB>::kStageMats;
+    __shared__ StreamRec<R, FULL, TILE> rec;
+    const int2 td = tiles[blockIdx.x];
+    const int start = td.x, cnt = td.y & 0xffff, code = td.y >> 16;
+    const double mscale = (MODE == MODE_A) ? -k0p.v[0] : 1.0;
 
-    for (int i = threadIdx.x; i < 64 * 20; i += kBlock) sm.slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
-
-    const long long tile0 = (long long)blockIdx.x * kTile;
-    const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
-    if (threadIdx.x < ntile)
-        tet_geometry<R, ANTI>(m, eps, mu, tile0 + threadIdx.x, threadIdx.x, sm.rec);
-
-    // Pair coefficients of this thread (registers).
-    const int j = threadIdx.x;
-    const bool active = j < NED2_NPAIR;
-    const int pj = active ? j : 0;
-    const int a = PAIR_A[pj], b = PAIR_B[pj];
-    R tk[6], tm[6], qk[3], qm[3];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        tk[c] = (R)TSYM_K[pj][c];
-        tm[c] = (R)TSYM_M[pj][c];
+    for (int lt = threadIdx.x; lt < cnt; lt += kBlock) {
+        const int ia = order[start + lt];
+        const long long t = m.tetids ? (long long)m.tetids[ia] : (long long)ia;
+        int c;
+        tet_geometry<R, FULL>(m, eps, mu, t, mscale, rec.gk[lt], rec.gm[lt],
+                              rec.ak[FULL ? lt : 0], rec.am[FULL ? lt : 0], c, rec.sgn[lt]);
+        rec.ia[lt] = ia;
     }
+
+    // the two output slots of this thread and their coefficients: complex64 entries go
+    // in adjacent pairs (one 16-byte store), complex128 entries i and i + 200 (two
+    // 16-byte stores, each contiguous across the wavefront)
+    const bool active = threadIdx.x < 200;
+    const int ti = active ? (int)threadIdx.x : 0;
+    constexpr bool kPaired = sizeof(R) == 4;
+    const int E[2] = {kPaired ? 2 * ti : ti, kPaired ? 2 * ti + 1 : ti + 200};
+    R tk[2][6], tm[2][6], qk[2][3], qm[2][3];
+    bool odd[2];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        qk[c] = ANTI ? (R)TANTI_K[pj][c] : (R)0;
-        qm[c] = ANTI ? (R)TANTI_M[pj][c] : (R)0;
+    for (int q = 0; q < 2; ++q) {
+        const int ii = E[q] / 20, jj = E[q] % 20;
+        const int ca = CANON[code][ii], cb = CANON[code][jj];
+        const int p = PAIR_INDEX[ca][cb];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            tk[q][c] = (R)TSYM_K[p][c];
+            tm[q][c] = (R)TSYM_M[p][c];
+        }
+        // antisymmetric sign: -1 when the pair's lower canonical function sits in the
+        // lower reference slot
+        const int slot_lo = (ca <= cb) ? ii : jj, slot_hi = (ca <= cb) ? jj : ii;
+        const R sa = (slot_lo < slot_hi) ? (R)-1 : (R)1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            qk[q][c] = FULL ? sa * (R)TANTI_K[p][c] : (R)0;
+            qm[q][c] = FULL ? sa * (R)TANTI_M[p][c] : (R)0;
+        }
+        odd[q] = (ii >= 6) != (jj >= 6);
     }
-    const bool aodd = a >= 6, bodd = b >= 6;
-    const R kq = (R)k0p.v[0];
     __syncthreads();
+    if (!active) return;
 
-    const int nsub = (ntile + NB - 1) / NB;
-    for (int sb = 0; sb < nsub; ++sb) {
-        const int buf = sb & 1;
-        const int lt0 = sb * NB;
-        const int nvalid = min(NB, ntile - lt0);
-        if (active) {
+    for (int lt = 0; lt < cnt; ++lt) {
+        R gkr[6], gki[6], gmr[6], gmi[6];
 #pragma unroll
-            for (int u = 0; u < NB; ++u) {
-                if (u < nvalid) {
-                    const int lt = lt0 + u;
-                    const int code = sm.rec.code[lt];
-                    const int ra = sm.slot[code][a], rb = sm.slot[code][b];
-                    R kr = 0, ki = 0, mr = 0, mi = 0;
+        for (int c = 0; c < 6; ++c) {
+            gkr[c] = rec.gk[lt][c][0];
+            gki[c] = rec.gk[lt][c][1];
+            gmr[c] = rec.gm[lt][c][0];
+            gmi[c] = rec.gm[lt][c][1];
+        }
+        const R s = rec.sgn[lt];
+        const long long base = (long long)rec.ia[lt] * 400;
+        R kr[2], ki[2], mr[2], mi[2];
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        kr += tk[c] * sm.rec.gk[lt][c][0];
-                        ki += tk[c] * sm.rec.gk[lt][c][1];
-                        mr += tm[c] * sm.rec.gm[lt][c][0];
-                        mi += tm[c] * sm.rec.gm[lt][c][1];
-                    }
-                    if constexpr (ANTI) {
-                        R xr = 0, xi = 0, yr = 0, yi = 0;
+        for (int q = 0; q < 2; ++q) {
+            R a = 0, b = 0, c2 = 0, d = 0;
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            xr += qk[c] * sm.rec.ak[lt][c][0];
-                            xi += qk[c] * sm.rec.ak[lt][c][1];
-                            yr += qm[c] * sm.rec.am[lt][c][0];
-                            yi += qm[c] * sm.rec.am[lt][c][1];
-                        }
-                        const R sa = (ra < rb) ? (R)-1 : (R)1;
-                        kr += sa * xr;
-                        ki += sa * xi;
-                        mr += sa * yr;
-                        mi += sa * yi;
-                    }
-                    const R s = sm.rec.sgn[lt];
-                    const R sg = (aodd ? s : (R)1) * (bodd ? s : (R)1);
-                    const int e1 = ra * 20 + rb, e2 = rb * 20 + ra;
-                    if (MODE == MODE_A) {
-                        C2<R> v = {sg * (kr - kq * mr), sg * (ki - kq * mi)};
-                        sm.stage[buf][0][u * 400 + e1] = v;
-                        sm.stage[buf][0][u * 400 + e2] = v;
-                    } else {
-                        C2<R> vk = {sg * kr, sg * ki}, vm = {sg * mr, sg * mi};
-                        sm.stage[buf][0][u * 400 + e1] = vk;
-                        sm.stage[buf][0][u * 400 + e2] = vk;
-                        sm.stage[buf][1][u * 400 + e1] = vm;
-                        sm.stage[buf][1][u * 400 + e2] = vm;
-                    }
+            for (int c = 0; c < 6; ++c) {
+                a += tk[q][c] * gkr[c];
+                b += tk[q][c] * gki[c];
+                c2 += tm[q][c] * gmr[c];
+                d += tm[q][c] * gmi[c];
+            }
+            if constexpr (FULL) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    a += qk[q][c] * rec.ak[lt][c][0];
+                    b += qk[q][c] * rec.ak[lt][c][1];
+                    c2 += qm[q][c] * rec.am[lt][c][0];
+                    d += qm[q][c] * rec.am[lt][c][1];
                 }
             }
+            const R sg = odd[q] ? s : (R)1;
+            kr[q] = sg * a;
+            ki[q] = sg * b;
+            mr[q] = sg * c2;
+            mi[q] = sg * d;
         }
-        __syncthreads();
-        const long long ia0 = tile0 + lt0;
-        const int nent = nvalid * 400;
         if (MODE == MODE_A) {
-            O* dst = reinterpret_cast<O*>(out0) + ia0 * 400;
-            stream_out(&sm.stage[buf][0][0], dst, nent * (int)sizeof(O));
+            // G_M already carries -k0^2
+            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0] + mr[0], ki[0] + mi[0],
+                   kr[1] + mr[1], ki[1] + mi[1]);
         } else if (MODE == MODE_KM) {
-            stream_out(&sm.stage[buf][0][0], reinterpret_cast<O*>(out0) + ia0 * 400,
-                       nent * (int)sizeof(O));
-            stream_out(&sm.stage[buf][1][0], reinterpret_cast<O*>(out1) + ia0 * 400,
-                       nent * (int)sizeof(O));
+            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0], ki[0], kr[1], ki[1]);
+            store2(reinterpret_cast<O*>(out1) + base, E[0], E[1], mr[0], mi[0], mr[1], mi[1]);
         } else {
-            // A_f = K - k0_f^2 M for every wavenumber, 16 bytes per store.
-            constexpr int per = 16 / (int)sizeof(O);  // complex entries per store
-            const int nvec = nent / per;
             for (int f = 0; f < nf; ++f) {
                 const R kf = (R)k0p.v[f];
-                u32x4* dst = reinterpret_cast<u32x4*>(reinterpret_cast<O*>(out0) +
-                                                      (long long)f * freq_stride + ia0 * 400);
-                for (int i = threadIdx.x; i < nvec; i += kBlock) {
-                    union {
-                        O o[per];
-                        u32x4 v;
-                    } pk;
-#pragma unroll
-                    for (int q = 0; q < per; ++q) {
-                        const C2<R> K = sm.stage[buf][0][i * per + q];
-                        const C2<R> M = sm.stage[buf][NS - 1][i * per + q];
-                        pk.o[q].x = K.re - kf * M.re;
-                        pk.o[q].y = K.im - kf * M.im;
-                    }
-                    __builtin_nontemporal_store(pk.v, dst + i);
-                }
+                store2(reinterpret_cast<O*>(out0) + (long long)f * freq_stride + base, E[0], E[1],
+                       kr[0] - kf * mr[0], ki[0] - kf * mi[0], kr[1] - kf * mr[1],
+                       ki[1] - kf * mi[1]);
             }
         }
-        // The next sub-batch writes the other buffer; the barrier above already ordered
-        // every read of that buffer (two sub-batches ago) before this point.
     }
 }
 
 // Launch over all tiles; MODE_AF launches once per chunk of up to kMaxK0 wavenumbers
 // (passed by value, so the launch needs no device buffer and can be graph-captured).
-template <typename R, int MODE, bool ANTI, int NB>
-static hipError_t launch_t(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
-                           const double* k0sq, int nf, void* out0, void* out1,
+template <typename R, int MODE, bool FULL, int TILE>
+static hipError_t launch_t(const MeshDev& m, const TileSet& ts, const MatDesc& eps,
+                           const MatDesc& mu, const double* k0sq, int nf, void* out0, void* out1,
                            long long freq_stride, hipStream_t stream) {
     using O = typename OutVec<R>::T;
-    const long long ntiles = (m.n_assy + kTile - 1) / kTile;
-    if (ntiles == 0) return hipSuccess;
+    if (ts.ntiles == 0) return hipSuccess;
     const int nchunk = (MODE == MODE_AF) ? nf : 1;
     for (int f0 = 0; f0 < nchunk; f0 += kMaxK0) {
         K0Pack p;
         const int n = (MODE == MODE_AF) ? min(kMaxK0, nf - f0) : 1;
         for (int f = 0; f < kMaxK0; ++f) p.v[f] = (k0sq && f < n) ? k0sq[f0 + f] : 0.0;
         void* o0 = reinterpret_cast<O*>(out0) + (long long)f0 * freq_stride;
-        hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, ANTI, NB>), dim3((unsigned)ntiles),
-                           dim3(kBlock), 0, stream, m, eps, mu, p, n, o0, out1, freq_stride);
+        hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, FULL, TILE>), dim3((unsigned)ts.ntiles),
+                           dim3(kBlock), 0, stream, m, eps, mu, p, n, ts.order, ts.tiles, o0, out1,
+                           freq_stride);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -453,56 +434,52 @@ static hipError_t launch_t(const MeshDev& m, const MatDesc& eps, const MatDesc& 
 }
 
 template <typename R, int MODE, bool FULL>
-static hipError_t launch_nb(const MeshDev& m, const MatDesc& eps, const MatDesc& mu,
-                            const double* k0sq, int nf, void* out0, void* out1,
-                            long long freq_stride, int nb, hipStream_t stream) {
-    // MODE_KM stages two matrices per tet: it keeps the smallest LDS footprint.
-    if constexpr (MODE == MODE_KM) {
-        return launch_t<R, MODE, FULL, 1>(m, eps, mu, k0sq, nf, out0, out1,
-                                          freq_stride, stream);
-    }
-    if (nb == 1)
-        return launch_t<R, MODE, FULL, 1>(m, eps, mu, k0sq, nf, out0, out1,
-                                          freq_stride, stream);
-    if (nb == 4)
-        return launch_t<R, MODE, FULL, 4>(m, eps, mu, k0sq, nf, out0, out1,
-                                          freq_stride, stream);
-    return launch_t<R, MODE, FULL, 2>(m, eps, mu, k0sq, nf, out0, out1, freq_stride,
-                                      stream);
+static hipError_t launch_tile(const MeshDev& m, const TileSet* ts, int tile, const MatDesc& eps,
+                              const MatDesc& mu, const double* k0sq, int nf, void* out0,
+                              void* out1, long long freq_stride, hipStream_t stream) {
+    if (tile == 32)
+        return launch_t<R, MODE, FULL, 32>(m, ts[0], eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                           stream);
+    if (tile == 128)
+        return launch_t<R, MODE, FULL, 128>(m, ts[2], eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                            stream);
+    return launch_t<R, MODE, FULL, 64>(m, ts[1], eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                       stream);
 }
 
 template <typename R, int MODE>
-static hipError_t launch_anti(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int anti,
-                              const double* k0sq, int nf, void* out0, void* out1,
-                              long long freq_stride, int nb, hipStream_t stream) {
-    if (anti)
-        return launch_nb<R, MODE, true>(m, eps, mu, k0sq, nf, out0, out1,
-                                        freq_stride, nb, stream);
-    return launch_nb<R, MODE, false>(m, eps, mu, k0sq, nf, out0, out1, freq_stride,
-                                     nb, stream);
+static hipError_t launch_full(const MeshDev& m, const TileSet* ts, int tile, const MatDesc& eps,
+                              const MatDesc& mu, int full, const double* k0sq, int nf, void* out0,
+                              void* out1, long long freq_stride, hipStream_t stream) {
+    if (full)
+        return launch_tile<R, MODE, true>(m, ts, tile, eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                          stream);
+    return launch_tile<R, MODE, false>(m, ts, tile, eps, mu, k0sq, nf, out0, out1, freq_stride,
+                                       stream);
 }
 
 template <typename R>
-static hipError_t launch_mode(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
-                              int anti, const double* k0sq, int nf, void* out0, void* out1,
-                              long long freq_stride, int nb, hipStream_t stream) {
+static hipError_t launch_mode(const MeshDev& m, const TileSet* ts, int tile, const MatDesc& eps,
+                              const MatDesc& mu, int mode, int full, const double* k0sq, int nf,
+                              void* out0, void* out1, long long freq_stride, hipStream_t stream) {
     if (mode == MODE_A)
-        return launch_anti<R, MODE_A>(m, eps, mu, anti, k0sq, 1, out0, out1, freq_stride, nb,
-                                      stream);
+        return launch_full<R, MODE_A>(m, ts, tile, eps, mu, full, k0sq, 1, out0, out1,
+                                      freq_stride, stream);
     if (mode == MODE_KM)
-        return launch_anti<R, MODE_KM>(m, eps, mu, anti, nullptr, 0, out0, out1, freq_stride, nb,
-                                       stream);
-    return launch_anti<R, MODE_AF>(m, eps, mu, anti, k0sq, nf, out0, out1, freq_stride, nb,
+        return launch_full<R, MODE_KM>(m, ts, tile, eps, mu, full, nullptr, 0, out0, out1,
+                                       freq_stride, stream);
+    return launch_full<R, MODE_AF>(m, ts, tile, eps, mu, full, k0sq, nf, out0, out1, freq_stride,
                                    stream);
 }
 
-hipError_t launch_assemble(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
-                           int precision, int anti, const double* k0sq, int nf, void* out0,
-                           void* out1, long long freq_stride, int nb, hipStream_t stream) {
+hipError_t launch_assemble(const MeshDev& m, const TileSet* ts, int tile, const MatDesc& eps,
+                           const MatDesc& mu, int mode, int precision, int full,
+                           const double* k0sq, int nf, void* out0, void* out1,
+                           long long freq_stride, hipStream_t stream) {
     if (precision == EM_PREC_FP64)
-        return launch_mode<double>(m, eps, mu, mode, anti, k0sq, nf, out0, out1, freq_stride, nb,
-                                   stream);
-    return launch_mode<float>(m, eps, mu, mode, anti, k0sq, nf, out0, out1, freq_stride, nb,
+        return launch_mode<double>(m, ts, tile, eps, mu, mode, full, k0sq, nf, out0, out1,
+                                   freq_stride, stream);
+    return launch_mode<float>(m, ts, tile, eps, mu, mode, full, k0sq, nf, out0, out1, freq_stride,
                               stream);
 }
 

@@ -98,12 +98,12 @@ int em_ned2_plan_destroy(em_ned2_plan_t plan);
 int em_ned2_plan_sizes(em_ned2_plan_t plan, int64_t* n_assy, int64_t* n_field,
                        int64_t* nnz_coo);
 /*
- * Launch shape of the assembly kernels of this plan: tets per LDS sub-batch (1, 2 or 4;
- * default 2). No counterpart in the reference (Numba picks its own schedule); set by
- * the project's autotuner (emerge_amd/autotune.py).
+ * Launch shape of the assembly kernels of this plan: tets per workgroup tile (32, 64 or
+ * 128; default 64). No counterpart in the reference (Numba picks its own schedule); set
+ * by the project's autotuner (emerge_amd/autotune.py).
  */
-int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t nb);
-int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* nb);
+int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t tile);
+int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* tile);
 
 /*
  * Element stiffness and mass blocks, reference-style (curlcurl.py:387-456):

@@ -217,13 +217,19 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         }
     }
 
-    // Group the assembly indices by vertex-rank code (stable counting sort) and cut each
-    // group into tiles of 32, 64 and 128 tets.
+    // Tiles for the assembly kernels. The assembly range is cut into windows of kWindow
+    // consecutive tets (neighbours share nodes); inside each window the tets are grouped
+    // by vertex-rank code (stable counting sort) and each group is cut into tiles of 32,
+    // 64 and 128 tets. Workgroups are dispatched round-robin over the XCDs, so the tile
+    // list is interleaved such that every tile of a window runs on the same XCD and its
+    // node/material reads hit that XCD's L2: tile j*kXcds + x is the j-th tile of XCD x,
+    // which takes the windows w with w % kXcds == x (shorter lists padded with empty tiles).
+    constexpr long long kWindow = 16384;
+    constexpr int kXcds = 8;
     std::vector<int> horder((size_t)n_assy);
     std::vector<std::vector<int2>> htiles(3);
     {
         std::vector<unsigned char> code((size_t)n_assy);
-        long long cnt[65] = {0};
         for (long long ia = 0; ia < n_assy; ++ia) {
             const int4 v = htets[tetids ? hids[ia] : ia];
             const int n[4] = {v.x, v.y, v.z, v.w};
@@ -231,18 +237,33 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
             for (int a = 0; a < 4; ++a)
                 r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
             code[ia] = (unsigned char)(r[0] * 16 + r[1] * 4 + r[2]);
-            cnt[code[ia] + 1]++;
         }
-        for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
-        std::vector<long long> cur(cnt, cnt + 64);
-        for (long long ia = 0; ia < n_assy; ++ia) horder[cur[code[ia]]++] = (int)ia;
+        const long long nwin = (n_assy + kWindow - 1) / kWindow;
+        std::vector<std::vector<std::vector<int2>>> per_xcd(3, std::vector<std::vector<int2>>(kXcds));
+        for (long long w = 0; w < nwin; ++w) {
+            const long long w0 = w * kWindow, w1 = std::min<long long>(n_assy, w0 + kWindow);
+            long long cnt[65] = {0};
+            for (long long ia = w0; ia < w1; ++ia) cnt[code[ia] + 1]++;
+            for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
+            std::vector<long long> cur(cnt, cnt + 64);
+            for (long long ia = w0; ia < w1; ++ia) horder[w0 + cur[code[ia]]++] = (int)ia;
+            for (int k = 0; k < 3; ++k) {
+                const int ts = em::kTileSizes[k];
+                for (int c = 0; c < 64; ++c)
+                    for (long long s0 = cnt[c]; s0 < cnt[c + 1]; s0 += ts) {
+                        const long long n = std::min<long long>(ts, cnt[c + 1] - s0);
+                        per_xcd[k][w % kXcds].push_back(
+                            make_int2((int)(w0 + s0), (int)(n | ((long long)c << 16))));
+                    }
+            }
+        }
         for (int k = 0; k < 3; ++k) {
-            const int ts = em::kTileSizes[k];
-            for (int c = 0; c < 64; ++c)
-                for (long long s0 = cnt[c]; s0 < cnt[c + 1]; s0 += ts) {
-                    const long long n = std::min<long long>(ts, cnt[c + 1] - s0);
-                    htiles[k].push_back(make_int2((int)s0, (int)(n | ((long long)c << 16))));
-                }
+            size_t longest = 0;
+            for (int x = 0; x < kXcds; ++x) longest = std::max(longest, per_xcd[k][x].size());
+            htiles[k].assign(longest * kXcds, make_int2(0, 0));  // empty tiles: count 0
+            for (int x = 0; x < kXcds; ++x)
+                for (size_t j = 0; j < per_xcd[k][x].size(); ++j)
+                    htiles[k][j * kXcds + x] = per_xcd[k][x][j];
         }
     }
 

@@ -37,9 +37,10 @@ struct K0Pack {
     double v[kMaxK0];  // k0^2
 };
 
-// Tets grouped by vertex-rank code: order[] lists assembly indices bucket by bucket
-// (ascending within a bucket), tiles[] = (first position in order, count | code << 16)
-// with at most `tile` tets each.
+// Tets grouped by vertex-rank code inside windows of consecutive tets: order[] lists
+// assembly indices window by window, bucket by bucket (ascending within a bucket);
+// tiles[] = (first position in order, count | code << 16) with at most `tile` tets each,
+// interleaved so that all tiles of a window land on one XCD; count 0 marks a padding tile.
 struct TileSet {
     const int* order = nullptr;
     const int2* tiles = nullptr;

@@ -311,6 +311,7 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
     __shared__ StreamRec<R, FULL, TILE> rec;
     const int2 td = tiles[blockIdx.x];
     const int start = td.x, cnt = td.y & 0xffff, code = td.y >> 16;
+    if (cnt == 0) return;  // padding tile (uniform across the workgroup)
     const double mscale = (MODE == MODE_A) ? -k0p.v[0] : 1.0;
 
     for (int lt = threadIdx.x; lt < cnt; lt += kBlock) {

@@ -83,6 +83,14 @@ _SIGS = {
     "em_csc_to_csc": (ctypes.c_int, [P, P, P, I32, P]),
     "em_csc_to_csc_combine": (ctypes.c_int, [P, P, P, ctypes.c_double, P, I32, P]),
     "em_ned2_backward": (ctypes.c_int, [P, P, ctypes.c_double, I32, P, I32, P, P]),
+    "em_ned2_pec_mask": (ctypes.c_int, [P, P, I64, P, I64, P, I64, P, P]),
+    "em_dof_compact": (ctypes.c_int, [I64, P, P, P, ctypes.POINTER(I64), P]),
+    "em_csc_select_count": (ctypes.c_int, [I64, P, P, P, I32, I32, P, I64, P,
+                                           ctypes.POINTER(I64), P]),
+    "em_csc_select_fill": (ctypes.c_int, [I64, P, P, P, I32, I32, P, P, P, P, P]),
+    "em_csc_map_restrict": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), P]),
+    "em_dof_gather": (ctypes.c_int, [P, I64, I64, P, P, I32, P]),
+    "em_dof_scatter": (ctypes.c_int, [P, I64, I64, P, P, I32, P]),
     "em_graph_begin_capture": (ctypes.c_int, [P]),
     "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "em_graph_launch": (ctypes.c_int, [P, P]),

@@ -300,12 +300,18 @@ class CscMap:
     values of every CSC entry on the GPU.
     """
 
-    def __init__(self, plan: Ned2Plan, stream=None):
+    def __init__(self, plan: Ned2Plan, stream=None, *, _handle=None, _device=None):
         L = _lib.lib()
-        h = ctypes.c_void_p()
-        check(L.em_csc_map_create(ctypes.byref(h), plan._h, _stream_ptr(stream)))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            check(L.em_csc_map_create(ctypes.byref(h), plan._h, _stream_ptr(stream)))
+            self.device = plan.device
+        else:
+            h = _handle
+            self.device = _device
         self._h = h
-        self.device = plan.device
+        # a restricted map (``restrict``) covers the free-DOF block only
+        self.restricted = _handle is not None
         n, nnz, nnz_coo = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(L.em_csc_map_sizes(h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nnz_coo)))
         self.N, self.nnz, self.nnz_coo = n.value, nnz.value, nnz_coo.value
@@ -337,6 +343,19 @@ class CscMap:
                 ptrs.append(None)
         check(_lib.lib().em_csc_map_export(self._h, *ptrs, _stream_ptr(stream)))
         return out
+
+    def restrict(self, sel, stream=None) -> "CscMap":
+        """Map of the free-DOF block ``A[:, solve_ids][solve_ids, :]`` (solver.py:2071):
+        its ``to_csc``/``to_csc_combine`` read the same COO values and write the block's
+        values directly. ``sel`` is a ``SolveIds`` of this map's DOFs."""
+        if self.restricted:
+            raise ValueError("map is already restricted")
+        if sel.n_field != self.N:
+            raise ValueError(f"solve ids are for {sel.n_field} DOFs, map has {self.N}")
+        h = ctypes.c_void_p()
+        check(_lib.lib().em_csc_map_restrict(self._h, ctypes.c_void_p(sel.new_index.data_ptr()),
+                                             sel.n_solve, ctypes.byref(h), _stream_ptr(stream)))
+        return CscMap(None, _handle=h, _device=self.device)
 
     def to_csc(self, data: torch.Tensor, out=None, stream=None) -> torch.Tensor:
         """CSC values of the COO values ``data`` (complex128 or complex64 on device)."""

@@ -571,6 +571,8 @@ struct em_csc_map_s {
     int device = 0;
     long long N = 0, nnz_coo = 0;
     em::CscBuild b;
+    bool restricted = false;  // made by em_csc_map_restrict: no perm/target/col_offsets export
+    long long n_perm = 0;     // COO contributors of the entries (nnz_coo for a full map)
 };
 
 static void free_csc(em_csc_map_s* c) {
@@ -638,6 +640,10 @@ int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64
         set_error("null csc map");
         return EM_EINVAL;
     }
+    if (map->restricted && (perm || target_csc || col_offsets)) {
+        set_error("csc_map_export: a restricted map exports indptr and indices only");
+        return EM_EUNSUPPORTED;
+    }
     DeviceGuard g(map->device);
     hipStream_t s = (hipStream_t)stream;
     const hipMemcpyKind k = hipMemcpyDeviceToDevice;
@@ -683,6 +689,147 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
         return EM_EINVAL;
     }
     return csc_apply(map, dataE, dataB, k0, data_csc, precision, stream);
+}
+
+// ---------------------------------------------------------------------------------------
+// PEC elimination and the solver hand-off
+// ---------------------------------------------------------------------------------------
+
+static int check_prec(int32_t precision) {
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("unknown precision");
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_ned2_pec_mask(em_ned2_plan_t plan, const int64_t* conductor_tets, int64_t n_cond,
+                     const int64_t* tri_to_field, int64_t n_tris, const int64_t* pec_tris,
+                     int64_t n_pec, uint8_t* keep, void* stream) {
+    if (!plan || !keep || n_cond < 0 || n_pec < 0 || n_tris < 0 || (n_cond && !conductor_tets) ||
+        (n_pec && (!pec_tris || !tri_to_field))) {
+        set_error("pec_mask: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    long long bad = 0;
+    EM_HIP(em::pec_mark(plan->t2f, plan->n_tets, (const long long*)conductor_tets, n_cond,
+                        (const long long*)tri_to_field, n_tris, (const long long*)pec_tris, n_pec,
+                        plan->n_field, keep, bad, (hipStream_t)stream));
+    if (bad) {
+        set_error(std::string("pec_mask: out-of-range ") +
+                  ((bad & 1) ? "conductor tet id" : (bad & 2) ? "PEC face id" : "face DOF"));
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_dof_compact(int64_t n, const uint8_t* keep, int64_t* new_index, int64_t* solve_ids,
+                   int64_t* n_solve, void* stream) {
+    if (n < 0 || !n_solve || (n > 0 && (!keep || !new_index || !solve_ids))) {
+        set_error("dof_compact: bad argument");
+        return EM_EINVAL;
+    }
+    long long ns = 0;
+    EM_HIP(em::dof_compact(n, keep, (long long*)new_index, (long long*)solve_ids, ns,
+                           (hipStream_t)stream));
+    *n_solve = ns;
+    return EM_OK;
+}
+
+int em_csc_select_count(int64_t n, const int64_t* indptr, const int64_t* indices,
+                        const void* data, int32_t precision, int32_t drop_zeros,
+                        const int64_t* new_index, int64_t n_solve, int64_t* sel_indptr,
+                        int64_t* nnz_sel, void* stream) {
+    if (int st = check_prec(precision)) return st;
+    if (n < 0 || n_solve < 0 || n_solve > n || !indptr || !sel_indptr || !nnz_sel ||
+        (n > 0 && (!indices || !new_index)) || (drop_zeros && !data)) {
+        set_error("csc_select_count: bad argument");
+        return EM_EINVAL;
+    }
+    long long nz = 0;
+    bool bad = false;
+    EM_HIP(em::csc_select_count(n, (const long long*)indptr, (const long long*)indices, data,
+                                precision == EM_PREC_MIXED, drop_zeros != 0,
+                                (const long long*)new_index, n_solve, (long long*)sel_indptr, nz,
+                                bad, (hipStream_t)stream));
+    if (bad) {
+        set_error("csc_select_count: row index out of range");
+        return EM_EINVAL;
+    }
+    *nnz_sel = nz;
+    return EM_OK;
+}
+
+int em_csc_select_fill(int64_t n, const int64_t* indptr, const int64_t* indices,
+                       const void* data, int32_t precision, int32_t drop_zeros,
+                       const int64_t* new_index, const int64_t* sel_indptr, int64_t* sel_indices,
+                       void* sel_data, void* stream) {
+    if (int st = check_prec(precision)) return st;
+    if (n < 0 || !indptr || !sel_indptr || (n > 0 && (!indices || !new_index || !sel_indices)) ||
+        (drop_zeros && !data) || (sel_data && !data)) {
+        set_error("csc_select_fill: bad argument");
+        return EM_EINVAL;
+    }
+    EM_HIP(em::csc_select_fill(n, (const long long*)indptr, (const long long*)indices,
+                               (sel_data || drop_zeros) ? data : nullptr,
+                               precision == EM_PREC_MIXED,
+                               drop_zeros != 0, (const long long*)new_index,
+                               (const long long*)sel_indptr, (long long*)sel_indices, sel_data,
+                               (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_csc_map_restrict(em_csc_map_t map, const int64_t* new_index, int64_t n_solve,
+                        em_csc_map_t* sub, void* stream) {
+    if (!map || !sub || n_solve < 0 || n_solve > map->N || (map->N > 0 && !new_index)) {
+        set_error("csc_map_restrict: bad argument");
+        return EM_EINVAL;
+    }
+    *sub = nullptr;
+    if (map->restricted) {
+        set_error("csc_map_restrict: the map is already restricted");
+        return EM_EUNSUPPORTED;
+    }
+    DeviceGuard g(map->device);
+    auto* c = new em_csc_map_s();
+    c->device = map->device;
+    c->N = n_solve;
+    c->nnz_coo = map->nnz_coo;
+    c->restricted = true;
+    long long n_perm = 0;
+    hipError_t e = em::csc_restrict(map->b, map->N, (const long long*)new_index, n_solve, c->b,
+                                    n_perm, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        delete c;
+        set_error(std::string("csc_map_restrict: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;
+    }
+    c->n_perm = n_perm;
+    *sub = c;
+    return EM_OK;
+}
+
+static int dof_move(const int64_t* ids, int64_t n_ids, int64_t ncols, const void* src, void* dst,
+                    int32_t precision, int scatter, void* stream) {
+    if (int st = check_prec(precision)) return st;
+    if (n_ids < 0 || ncols < 0 || (n_ids * ncols > 0 && (!ids || !src || !dst))) {
+        set_error("dof gather/scatter: bad argument");
+        return EM_EINVAL;
+    }
+    EM_HIP(em::dof_move((const long long*)ids, n_ids, ncols, src, dst,
+                        precision == EM_PREC_MIXED, scatter, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_dof_gather(const int64_t* ids, int64_t n_ids, int64_t ncols, const void* src, void* dst,
+                  int32_t precision, void* stream) {
+    return dof_move(ids, n_ids, ncols, src, dst, precision, 0, stream);
+}
+
+int em_dof_scatter(const int64_t* ids, int64_t n_ids, int64_t ncols, const void* src, void* dst,
+                   int32_t precision, void* stream) {
+    return dof_move(ids, n_ids, ncols, src, dst, precision, 1, stream);
 }
 
 }  // extern "C"

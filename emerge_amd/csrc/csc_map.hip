@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_add_kernel(long long* __res
 }
 
 // Exclusive scan of n int64 values (in may equal out). Temporaries are allocated here.
-static hipError_t exclusive_scan(const long long* in, long long* out, long long n, hipStream_t s) {
+hipError_t exclusive_scan_i64(const long long* in, long long* out, long long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const long long ntiles = (n + kScanTile - 1) / kScanTile;
     long long *sums = nullptr, *offs = nullptr;
@@ -80,7 +80,7 @@ static hipError_t exclusive_scan(const long long* in, long long* out, long long 
         e = hipGetLastError();
     }
     if (e == hipSuccess && ntiles > 1) {
-        e = exclusive_scan(sums, offs, ntiles, s);
+        e = exclusive_scan_i64(sums, offs, ntiles, s);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s,
                                out, n, offs);
@@ -298,7 +298,7 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
                            m.tetids, m.n_assy, cnt);
         TRY(hipGetLastError());
     }
-    TRY(exclusive_scan((const long long*)cnt, b.inc_ptr, N + 1, s));
+    TRY(exclusive_scan_i64((const long long*)cnt, b.inc_ptr, N + 1, s));
     TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
     if (m.n_assy > 0) {
         hipLaunchKernelGGL(inc_fill_kernel, dim3(grid_for(m.n_assy * 20)), dim3(256), 0, s, m.t2f,
@@ -329,7 +329,7 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
             TRY(hipGetLastError());
         }
         TRY(hipMalloc((void**)&b.indptr, sizeof(long long) * (N + 1)));
-        TRY(exclusive_scan(ucount, b.indptr, N + 1, s));
+        TRY(exclusive_scan_i64(ucount, b.indptr, N + 1, s));
         TRY(hipMemcpyAsync(&b.nnz, b.indptr + N, sizeof(long long), hipMemcpyDeviceToHost, s));
         TRY(hipStreamSynchronize(s));
         TRY(hipMalloc((void**)&b.indices, sizeof(long long) * (b.nnz > 0 ? b.nnz : 1)));

@@ -83,6 +83,32 @@ hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz
                       const void* B, double k0sq, void* out, int complex64, hipStream_t s);
 hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s);
 hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s);
+// Exclusive scan of n int64 values (in may equal out); allocates temporaries, blocks.
+hipError_t exclusive_scan_i64(const long long* in, long long* out, long long n, hipStream_t s);
+
+// PEC elimination (dof_select.hip). keep[d] = 0 for prescribed DOFs; `bad` gets a bit set
+// for out-of-range ids (1 conductor tet, 2 face, 4 DOF). Blocks.
+hipError_t pec_mark(const int* t2f, long long n_tets, const long long* cond, long long n_cond,
+                    const long long* tri_to_field, long long n_tris, const long long* pec_tris,
+                    long long n_pec, long long n_field, unsigned char* keep, long long& bad,
+                    hipStream_t s);
+// new_index[d] = position of d among the kept DOFs or -1; solve_ids = kept DOFs ascending.
+hipError_t dof_compact(long long n, const unsigned char* keep, long long* new_index,
+                       long long* solve_ids, long long& n_solve, hipStream_t s);
+hipError_t csc_select_count(long long n, const long long* indptr, const long long* indices,
+                            const void* data, int complex64, int drop_zeros,
+                            const long long* new_index, long long n_solve, long long* sel_indptr,
+                            long long& nnz_sel, bool& bad_rows, hipStream_t s);
+hipError_t csc_select_fill(long long n, const long long* indptr, const long long* indices,
+                           const void* data, int complex64, int drop_zeros,
+                           const long long* new_index, const long long* sel_indptr,
+                           long long* sel_indices, void* sel_data, hipStream_t s);
+// Map of the kept rows/columns of `full` (N columns). sub.inc_ptr stays null; n_perm is
+// the number of COO contributors kept.
+hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_index,
+                        long long n_solve, CscBuild& sub, long long& n_perm, hipStream_t s);
+hipError_t dof_move(const long long* ids, long long n_ids, long long ncols, const void* src,
+                    void* dst, int complex64, int scatter, hipStream_t s);
 
 void set_error(const std::string& msg);
 

@@ -33,6 +33,8 @@ FACE_SLOTS = ((0, 1, 2), (0, 2, 3), (0, 3, 1), (1, 2, 3))
 DOFCODES_NED2 = np.array([64] * 6 + [66] * 6 + [128] * 4 + [129] * 4, dtype=np.int64)
 
 
+MISSING_ID = -1234  # reference mesh3d.py:29
+
 class MeshError(ValueError):
     """Raised for malformed or degenerate meshes (reference: ``MeshException``)."""
 
@@ -143,6 +145,39 @@ class TetMesh:
         """det[v1-v0, v2-v0, v3-v0] per tet in stored vertex order."""
         return tet_dets(self.nodes, self.tets)
 
+    @property
+    def tri_to_edge(self) -> np.ndarray:
+        """(3, F): edges (t0,t1), (t1,t2), (t0,t2) of each face's sorted node tuple
+        (reference mesh3d.py:832-839)."""
+        if getattr(self, "_tri_to_edge", None) is None:
+            nn = self.n_nodes
+            ekey = self.edges[0] * nn + self.edges[1]
+            t = self.tris
+            self._tri_to_edge = np.stack([np.searchsorted(ekey, t[a] * nn + t[b])
+                                          for a, b in ((0, 1), (1, 2), (0, 2))])
+        return self._tri_to_edge
+
+    @property
+    def tri_to_tet(self) -> np.ndarray:
+        """(2, F): the tets sharing each face, lower tet id first; ``MISSING_ID`` in row 1
+        for boundary faces (reference mesh3d.py:780-795)."""
+        if getattr(self, "_tri_to_tet", None) is None:
+            flat = self.tet_to_tri.ravel(order="F")
+            tet = np.repeat(np.arange(self.n_tets, dtype=np.int64), 4)
+            srt = np.argsort(flat, kind="stable")
+            ft, tt = flat[srt], tet[srt]
+            _, first, counts = np.unique(ft, return_index=True, return_counts=True)
+            out = np.full((2, self.n_tris), MISSING_ID, dtype=np.int64)
+            out[0, ft[first]] = tt[first]
+            two = counts == 2
+            out[1, ft[first[two]]] = tt[first[two] + 1]
+            self._tri_to_tet = out
+        return self._tri_to_tet
+
+    def boundary_tris(self) -> np.ndarray:
+        """Ids of the faces that belong to one tet only (the outer surface)."""
+        return np.flatnonzero(self.tri_to_tet[1] == MISSING_ID)
+
 
 class Nedelec2Field:
     """The DOF map of the second-order mixed Nedelec space on a ``TetMesh``.
@@ -175,6 +210,22 @@ class Nedelec2Field:
         t2f[16:20] = mesh.tet_to_tri + F + 2 * E
         self.tet_to_field = t2f
         self.n_tet_dofs = 20
+        self.n_tri_dofs = 8
+
+    @property
+    def edge_to_field(self) -> np.ndarray:
+        """(2, E): the two DOFs of each edge, e and e + E (nedelec2.py:70-73)."""
+        E = self.n_edges
+        return np.stack([np.arange(E), np.arange(E) + E]).astype(np.int64)
+
+    @property
+    def tri_to_field(self) -> np.ndarray:
+        """(8, F): the DOFs of each face's closure, rows 0-2 its edges' first DOFs, 3-5
+        their second DOFs, 6-7 the two face DOFs (nedelec2.py:70-79)."""
+        E, F = self.n_edges, self.n_tris
+        t2e = self.mesh.tri_to_edge
+        f = np.arange(F, dtype=np.int64)
+        return np.concatenate([t2e, t2e + E, (f + 2 * E)[None], (f + F + 2 * E)[None]])
 
     def get_tet_to_field(self) -> np.ndarray:
         return self.tet_to_field

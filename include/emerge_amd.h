@@ -173,6 +173,61 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
                           void* data_csc, int32_t precision, void* stream);
 
 /*
+ * PEC elimination and the solver hand-off. All pointers are DEVICE pointers; functions
+ * that take no plan run on the calling thread's current HIP device.
+ *
+ * em_ned2_pec_mask: the prescribed DOFs of Assembler.assemble_freq_matrix
+ * (assembler.py:533-564): keep[d] = 0 for the 20 DOFs of every conductor tet (plan tet
+ * ids) and for the DOFs of every PEC face: the face's column of the field's tri_to_field
+ * table (int64 [8][n_tris], nedelec2.py:70-79: its 3 edges' 2 DOFs each, then its 2 face
+ * DOFs; the edge DOFs are the reference's edge_to_field of the face's edges), keep[d] = 1
+ * otherwise. Out-of-range ids give EM_EINVAL. Blocks.
+ */
+int em_ned2_pec_mask(em_ned2_plan_t plan, const int64_t* conductor_tets, int64_t n_cond,
+                     const int64_t* tri_to_field, int64_t n_tris, const int64_t* pec_tris,
+                     int64_t n_pec, uint8_t* keep, void* stream);
+/*
+ * solve_ids = the kept DOFs ascending (assembler.py:742); new_index[d] = position of d in
+ * solve_ids, or -1. solve_ids needs room for n entries; *n_solve gets the count. Blocks.
+ */
+int em_dof_compact(int64_t n, const uint8_t* keep, int64_t* new_index, int64_t* solve_ids,
+                   int64_t* n_solve, void* stream);
+/*
+ * The free-DOF block A[:, solve_ids][solve_ids, :] of an n x n CSC matrix with sorted row
+ * indices (solver.py:2071; solver.py:2229-2230 np.ix_ is the same block). Two calls:
+ * _count writes sel_indptr [n_solve+1] and *nnz_sel (blocks); _fill writes sel_indices
+ * [nnz_sel] and, when sel_data is not NULL, sel_data (complex128 or complex64 per
+ * precision). drop_zeros also drops explicit zeros (K.eliminate_zeros(), assembler.py:760),
+ * which needs data in both calls.
+ */
+int em_csc_select_count(int64_t n, const int64_t* indptr, const int64_t* indices,
+                        const void* data, int32_t precision, int32_t drop_zeros,
+                        const int64_t* new_index, int64_t n_solve, int64_t* sel_indptr,
+                        int64_t* nnz_sel, void* stream);
+int em_csc_select_fill(int64_t n, const int64_t* indptr, const int64_t* indices,
+                       const void* data, int32_t precision, int32_t drop_zeros,
+                       const int64_t* new_index, const int64_t* sel_indptr, int64_t* sel_indices,
+                       void* sel_data, void* stream);
+/*
+ * A CSC map of the free-DOF block: em_csc_to_csc[_combine] on *sub read the same COO value
+ * arrays as on `map` and write the block's values directly (the reference's
+ * to_csc(Evec - Bvec K0^2) followed by A[:, solve_ids][solve_ids, :], fused). em_csc_map_sizes
+ * gives n = n_solve and the block's nnz; em_csc_map_export gives its indptr and indices
+ * only. Destroy with em_csc_map_destroy. Blocks.
+ */
+int em_csc_map_restrict(em_csc_map_t map, const int64_t* new_index, int64_t n_solve,
+                        em_csc_map_t* sub, void* stream);
+/*
+ * Row gather dst[i, :] = src[ids[i], :] (bsel = b[solve_ids, :], solver.py:2072) and
+ * scatter dst[ids[i], :] = src[i, :] (solution[solve_ids, :] = x, solver.py:2120, 2240-2241); row-major
+ * [rows][ncols], complex128 or complex64 per precision. Rows of dst not named are untouched.
+ */
+int em_dof_gather(const int64_t* ids, int64_t n_ids, int64_t ncols, const void* src, void* dst,
+                  int32_t precision, void* stream);
+int em_dof_scatter(const int64_t* ids, int64_t n_ids, int64_t ncols, const void* src, void* dst,
+                   int32_t precision, void* stream);
+
+/*
  * HIP graphs of assembly launches. The reference re-runs its Numba builder per call
  * (microwave_3d.py:1457 sweep loop) and has no counterpart: these record the launches
  * issued on `stream` between begin and end (hipStreamBeginCapture, thread-local mode)

@@ -72,13 +72,14 @@ def rel_err_chunked(a32, a64, chunk=1 << 26) -> float:
     return (num / den) ** 0.5
 
 
-def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int):
+def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int, nthreads: int = 0):
     """The oracle (C/OpenMP restatement of _matrix_builder, K and M plus int64 rows/cols
     like the reference) on the first ``chunk`` tets of the mesh, repeated for about
-    ``seconds``; returns tets/s and the thread count."""
+    ``seconds``; returns tets/s and the thread count (default: up to 16, the host share of
+    one GPU)."""
     import numpy as np
     import oracle
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
     m = field.mesh
     ids = np.arange(min(chunk, field.n_tets), dtype=np.int64)
     args = (m.nodes, m.tets, m.tris, m.edges, field.tet_to_field, m.tet_to_edge, m.tet_to_tri,
@@ -202,6 +203,94 @@ def suite_backward(args) -> dict:
             "leaves": "eps_r (T,) complex, mu_r^-1 (T,3) complex"}
 
 
+def suite_csc_pec(args) -> dict:
+    """COO -> CSC and PEC elimination on a ~500k-tet Delaunay mesh (all outer faces PEC):
+    one-time CSC map / PEC mask / restricted map builds, then per frequency the fused
+    to_csc(E - k0^2 B) of the full matrix, the free-DOF block selected from it, and the
+    restricted map that writes the block straight from the COO values (complex64).
+    The CPU line is SciPy's A[:, solve_ids][solve_ids, :] (the reference's own call,
+    solver.py:2071) on the same matrix."""
+    import numpy as np
+    import torch
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    from emerge_amd.pec import pec_solve_ids, select_csc, to_scipy
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=args.seed + 7))
+    plan = Ned2Plan.from_field(f)
+    T, N = plan.n_tets, f.n_field
+    eps, mu = _mats(T, np.random.default_rng(args.seed + 7))
+    K, M = plan.assemble_km(eps, mu, "mixed")
+    k0 = 2 * np.pi * 10e9 / C0
+
+    def once(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) * 1e3
+
+    cm, ms_map = once(lambda: CscMap(plan))
+    tris = f.mesh.boundary_tris()
+    sel, ms_mask = once(lambda: pec_solve_ids(plan, f, [], tris))
+    sub, ms_restrict = once(lambda: cm.restrict(sel))
+    log(f"csc/pec: {T} tets, N={N}, nnz={cm.nnz}, free DOFs {sel.n_solve}, block nnz {sub.nnz}")
+    full = torch.empty(cm.nnz, dtype=torch.complex64, device="cuda")
+    blk = torch.empty(sub.nnz, dtype=torch.complex64, device="cuda")
+    ex = cm.export("indptr", "indices")
+    ms_full, _ = time_launches(lambda: cm.to_csc_combine(K, M, k0, out=full), 10, 2)
+    ms_sel, _ = time_launches(lambda: select_csc(ex["indptr"], ex["indices"], full, sel), 5, 1)
+    ms_sub, _ = time_launches(lambda: sub.to_csc_combine(K, M, k0, out=blk), 10, 2)
+    # scalar eps_r and diagonal mu_r: symmetric blocks, mirrored-read gather
+    ms_full_m, _ = time_launches(
+        lambda: cm.to_csc_combine(K, M, k0, out=full, block_symmetric=True), 10, 2)
+    ms_sub_m, _ = time_launches(
+        lambda: sub.to_csc_combine(K, M, k0, out=blk, block_symmetric=True), 10, 2)
+    same_m = bool(torch.equal(sub.to_csc_combine(K, M, k0),
+                              sub.to_csc_combine(K, M, k0, block_symmetric=True)))
+    ref_blk = select_csc(ex["indptr"], ex["indices"], cm.to_csc_combine(K, M, k0), sel)[2]
+    sub.to_csc_combine(K, M, k0, out=blk)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(ref_blk, blk))
+    del ref_blk
+    # algorithmic bytes of a gather: per COO contributor perm (8) + E, B (8 + 8), per CSC
+    # entry seg (8) + value out (8). Contributors of the block: COO entries whose CSC entry
+    # has a free row and a free column.
+    tgt = cm.export("target_csc")["target_csc"]
+    keep = sel.keep.bool()
+    col = torch.repeat_interleave(torch.arange(N, device="cuda"), torch.diff(ex["indptr"]))
+    kept_q = keep[ex["indices"]] & keep[col]
+    n_contrib = int(kept_q[tgt].sum())
+    del tgt, col, kept_q
+    A_h = to_scipy(ex["indptr"], ex["indices"], full.to(torch.complex128), N)
+    ids = sel.solve_ids.cpu().numpy()
+    t0 = time.perf_counter()
+    blk_h = A_h[:, ids][ids, :]
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    assert blk_h.nnz == sub.nnz
+    del A_h, blk_h
+    res = {"n_tets": T, "n_dofs": N, "nnz": cm.nnz, "n_solve": sel.n_solve,
+           "block_nnz": sub.nnz,
+           "one_time_ms": {"csc_map": ms_map, "pec_mask_and_solve_ids": ms_mask,
+                           "restricted_map": ms_restrict},
+           "per_frequency_ms": {"to_csc_combine_full": ms_full, "select_block_from_full": ms_sel,
+                                "restricted_to_csc_combine": ms_sub,
+                                "to_csc_combine_full_mirrored": ms_full_m,
+                                "restricted_to_csc_combine_mirrored": ms_sub_m},
+           "restricted_equals_selected_full": same,
+           "mirrored_equals_plain": same_m,
+           "cpu_scipy_select_ms": cpu_ms,
+           "block_coo_contributors": n_contrib,
+           "full_gather_gb_per_s": (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full * 1e-3) / 1e9,
+           "restricted_gather_gb_per_s": (n_contrib * 24 + sub.nnz * 16) / (ms_sub * 1e-3) / 1e9,
+           "full_gather_mirrored_gb_per_s":
+               (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full_m * 1e-3) / 1e9,
+           "restricted_gather_mirrored_gb_per_s":
+               (n_contrib * 24 + sub.nnz * 16) / (ms_sub_m * 1e-3) / 1e9}
+    del full, blk, K, M, cm, sub
+    torch.cuda.empty_cache()
+    return res
+
+
 def target_run(args) -> dict:
     """The large single-mesh case: mixed and FP64 fused A on a ~8M-tet Delaunay mesh."""
     import numpy as np
@@ -264,6 +353,8 @@ def main() -> int:
     ap.add_argument("--suite", action="store_true",
                     help="also measure the frequency sweep (500k tets x 64 f), the uneven "
                          "32-mesh batch and the backward on 1M tets")
+    ap.add_argument("--only", default="",
+                    help="with --suite: comma list among sweep,batch,backward,csc_pec")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -355,6 +446,11 @@ def main() -> int:
                          f"({done} tets in {el:.1f}s); K and M complex128 + int64 rows/cols "
                          f"as the reference's _matrix_builder"}
         log(f"cpu: {tps:.0f} tets/s on {nth} threads")
+        # the reference's default thread count (NUMBA_NUM_THREADS=4)
+        tps4, _, done4, el4 = cpu_baseline(field, er_ref, ur_ref, args.cpu_seconds / 2,
+                                           args.cpu_chunk, nthreads=4)
+        cpu["value_4_threads"] = tps4
+        log(f"cpu: {tps4:.0f} tets/s on 4 threads")
 
     out = {
         "metric": METRIC,
@@ -391,12 +487,15 @@ def main() -> int:
         torch.cuda.empty_cache()
         extra = {}
         if args.suite:
-            log("suite: frequency sweep")
-            extra["sweep_500k_x64"] = suite_sweep(args)
-            log("suite: uneven 32-mesh batch")
-            extra["batch_32_meshes"] = suite_batch(args)
-            log("suite: backward")
-            extra["backward_1M"] = suite_backward(args)
+            runs = (("sweep", "sweep_500k_x64", suite_sweep),
+                    ("batch", "batch_32_meshes", suite_batch),
+                    ("backward", "backward_1M", suite_backward),
+                    ("csc_pec", "csc_pec_500k", suite_csc_pec))
+            only = set(args.only.split(",")) if args.only else None
+            for short, key, fn in runs:
+                if only is None or short in only:
+                    log(f"suite: {short}")
+                    extra[key] = fn(args)
         if args.target_points:
             extra["target"] = target_run(args)
         out["configs"] = extra

@@ -658,7 +658,11 @@ int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64
 }
 
 static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, void* out,
-                     int32_t precision, void* stream) {
+                     int32_t precision, int32_t flags, void* stream) {
+    if (flags & ~EM_CSC_BLOCK_SYMMETRIC) {
+        set_error("unknown to_csc flags");
+        return EM_EINVAL;
+    }
     if (!map) {
         set_error("null csc map");
         return EM_EINVAL;
@@ -673,13 +677,14 @@ static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, 
     }
     DeviceGuard g(map->device);
     EM_HIP(em::csc_gather(map->b.seg, map->b.perm, map->b.nnz, E, B, k0 * k0, out,
-                          precision == EM_PREC_MIXED ? 1 : 0, (hipStream_t)stream));
+                          precision == EM_PREC_MIXED ? 1 : 0,
+                          (flags & EM_CSC_BLOCK_SYMMETRIC) ? 1 : 0, (hipStream_t)stream));
     return EM_OK;
 }
 
 int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_t precision,
                   void* stream) {
-    return csc_apply(map, data_coo, nullptr, 0.0, data_csc, precision, stream);
+    return csc_apply(map, data_coo, nullptr, 0.0, data_csc, precision, 0, stream);
 }
 
 int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
@@ -688,7 +693,12 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
         set_error("to_csc_combine: null dataB");
         return EM_EINVAL;
     }
-    return csc_apply(map, dataE, dataB, k0, data_csc, precision, stream);
+    return csc_apply(map, dataE, dataB, k0, data_csc, precision, 0, stream);
+}
+
+int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                     void* data_csc, int32_t precision, int32_t flags, void* stream) {
+    return csc_apply(map, dataE, dataB, k0, data_csc, precision, flags, stream);
 }
 
 // ---------------------------------------------------------------------------------------

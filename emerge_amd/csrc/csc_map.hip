@@ -226,8 +226,19 @@ __device__ __forceinline__ void store_c<float2>(float2* p, long long q, double2 
     p[q] = make_float2((float)v.x, (float)v.y);
 }
 
+// COO index of the mirrored entry (jj, ii) of entry k = 400 ia + 20 ii + jj.
+__device__ __forceinline__ long long mirror_coo(long long k) {
+    const long long blk = k / 400, e = k - blk * 400;
+    const long long ii = e / 20, jj = e - ii * 20;
+    return blk * 400 + jj * 20 + ii;
+}
+
 // data_csc[q] = sum over contributors of (E[k] - k0sq * B[k]) (B may be null: E[k]).
-template <typename T>
+// MIRROR: the blocks are symmetric, so read entry k from its mirror. The contributors of
+// one column then lie in contiguous 20-entry rows of the blocks (one per incident tet)
+// instead of 20-entry strides, and a wavefront's reads share cache lines; the values and
+// their summation order are unchanged.
+template <typename T, bool MIRROR>
 __global__ __launch_bounds__(256) void csc_gather_kernel(const long long* __restrict__ seg,
                                                          const long long* __restrict__ perm,
                                                          long long nnz, const T* __restrict__ E,
@@ -238,7 +249,7 @@ __global__ __launch_bounds__(256) void csc_gather_kernel(const long long* __rest
         const long long j0 = seg[q], j1 = seg[q + 1];
         double2 acc = make_double2(0.0, 0.0);
         for (long long j = j0; j < j1; ++j) {
-            const long long k = perm[j];
+            const long long k = MIRROR ? mirror_coo(perm[j]) : perm[j];
             double2 v = LoadC<T>::get(E, k);
             if (B) {
                 const double2 b = LoadC<T>::get(B, k);
@@ -362,15 +373,25 @@ fail:
 #undef TRY
 }
 
+template <typename T>
+static void launch_gather(bool mirror, const long long* seg, const long long* perm, long long nnz,
+                          const void* E, const void* B, double k0sq, void* out, hipStream_t s) {
+    if (mirror)
+        hipLaunchKernelGGL((csc_gather_kernel<T, true>), dim3(grid_for(nnz)), dim3(256), 0, s, seg,
+                           perm, nnz, (const T*)E, (const T*)B, k0sq, (T*)out);
+    else
+        hipLaunchKernelGGL((csc_gather_kernel<T, false>), dim3(grid_for(nnz)), dim3(256), 0, s,
+                           seg, perm, nnz, (const T*)E, (const T*)B, k0sq, (T*)out);
+}
+
 hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz, const void* E,
-                      const void* B, double k0sq, void* out, int complex64, hipStream_t s) {
+                      const void* B, double k0sq, void* out, int complex64, int mirror,
+                      hipStream_t s) {
     if (nnz == 0) return hipSuccess;
     if (complex64)
-        hipLaunchKernelGGL(csc_gather_kernel<float2>, dim3(grid_for(nnz)), dim3(256), 0, s, seg,
-                           perm, nnz, (const float2*)E, (const float2*)B, k0sq, (float2*)out);
+        launch_gather<float2>(mirror != 0, seg, perm, nnz, E, B, k0sq, out, s);
     else
-        hipLaunchKernelGGL(csc_gather_kernel<double2>, dim3(grid_for(nnz)), dim3(256), 0, s, seg,
-                           perm, nnz, (const double2*)E, (const double2*)B, k0sq, (double2*)out);
+        launch_gather<double2>(mirror != 0, seg, perm, nnz, E, B, k0sq, out, s);
     return hipGetLastError();
 }
 

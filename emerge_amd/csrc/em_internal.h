@@ -79,8 +79,10 @@ struct CscBuild {
 };
 hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
                      hipStream_t s);
+// mirror: the COO blocks are symmetric; read each contributor from its mirrored entry.
 hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz, const void* E,
-                      const void* B, double k0sq, void* out, int complex64, hipStream_t s);
+                      const void* B, double k0sq, void* out, int complex64, int mirror,
+                      hipStream_t s);
 hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s);
 hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s);
 // Exclusive scan of n int64 values (in may equal out); allocates temporaries, blocks.

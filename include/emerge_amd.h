@@ -171,6 +171,18 @@ int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_
 /* Fused to_csc(dataE - k0^2 * dataB) (assembler.py:494) without forming the COO sum. */
 int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                           void* data_csc, int32_t precision, void* stream);
+/*
+ * Both of the above with flags (dataB NULL: to_csc of dataE). EM_CSC_BLOCK_SYMMETRIC
+ * promises that every 20x20 block of the COO values is symmetric; each contributor is
+ * then read from its mirrored entry, so a column's contributors are contiguous rows of
+ * the blocks (far fewer cache lines per column). The assembly with scalar or diagonal
+ * tensors makes entries (ii, jj) and (jj, ii) bit-identical, and the result is then the
+ * same bit for bit; symmetric full tensors give blocks symmetric to FP64 rounding, and a
+ * result equal to that level.
+ */
+#define EM_CSC_BLOCK_SYMMETRIC 1
+int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                     void* data_csc, int32_t precision, int32_t flags, void* stream);
 
 /*
  * PEC elimination and the solver hand-off. All pointers are DEVICE pointers; functions

@@ -96,3 +96,45 @@ def test_dropin_matches_reference_pipeline(wr90_field):
     # second call reuses the map (the reference caches it, assembler.py:490-491)
     E2, B2, c2 = tet_mass_stiffness_matrices(wr90_field, er, ur, conductor, cscmap)
     assert c2 is cscmap and np.array_equal(E2, Evec)
+
+
+@pytest.mark.parametrize("kinds", [("scalar", "diag"), ("diag", "diag"), ("scalar", "scalar")])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_blocks_symmetric_and_mirrored_gather(delaunay_field, kinds, precision):
+    """Scalar and diagonal tensors give bit-symmetric 20x20 blocks, so the mirrored-read
+    gather (EM_CSC_BLOCK_SYMMETRIC) returns exactly the plain gather's values, for
+    to_csc and the fused combine."""
+    import torch
+    from emerge_amd.assembly import CscMap
+    rng = np.random.default_rng(22)
+    T = delaunay_field.n_tets
+    er, ur = ref_tensor(T, kinds[0], rng), ref_tensor(T, kinds[1], rng)
+    plan = _field_plan(delaunay_field)
+    cm = CscMap(plan)
+    k0 = 2 * np.pi * 9.5e9 / 299792458.0
+    K, M = plan.assemble_km(er, ur, precision)
+    A = plan.assemble_a(er, ur, k0, precision)
+    for X in (K, M, A):
+        blk = X.view(T, 20, 20)
+        assert torch.equal(blk, blk.transpose(1, 2))
+    assert torch.equal(cm.to_csc(A), cm.to_csc(A, block_symmetric=True))
+    assert torch.equal(cm.to_csc_combine(K, M, k0),
+                       cm.to_csc_combine(K, M, k0, block_symmetric=True))
+
+
+def test_mirrored_gather_symmetric_full_tensors(delaunay_field):
+    """Symmetric full tensors: the blocks are symmetric to rounding (the general path
+    forms G and its antisymmetric part in FP64), and the mirrored gather agrees with the
+    plain one to that level."""
+    import torch
+    from emerge_amd.assembly import CscMap
+    rng = np.random.default_rng(23)
+    T = delaunay_field.n_tets
+    er, ur = ref_tensor(T, "sym", rng), ref_tensor(T, "sym", rng)
+    plan = _field_plan(delaunay_field)
+    cm = CscMap(plan)
+    A = plan.assemble_a(er, ur, 2 * np.pi * 9.5e9 / 299792458.0, "fp64")
+    blk = A.view(T, 20, 20)
+    assert float(torch.linalg.norm(blk - blk.transpose(1, 2)) / torch.linalg.norm(blk)) < 1e-15
+    a, b = cm.to_csc(A), cm.to_csc(A, block_symmetric=True)
+    assert float(torch.linalg.norm(a - b) / torch.linalg.norm(a)) < 1e-15

@@ -247,6 +247,16 @@ def suite_csc_pec(args) -> dict:
         lambda: sub.to_csc_combine(K, M, k0, out=blk, block_symmetric=True), 10, 2)
     same_m = bool(torch.equal(sub.to_csc_combine(K, M, k0),
                               sub.to_csc_combine(K, M, k0, block_symmetric=True)))
+    # the full map's default is the per-column incidence gather; the per-entry segment
+    # gather (what restricted maps use) beside it
+    ms_full_seg, _ = time_launches(
+        lambda: cm.to_csc_combine(K, M, k0, out=full, segment_gather=True), 10, 2)
+    ms_full_seg_m, _ = time_launches(
+        lambda: cm.to_csc_combine(K, M, k0, out=full, block_symmetric=True,
+                                  segment_gather=True), 10, 2)
+    same_inc = bool(torch.equal(
+        cm.to_csc_combine(K, M, k0, block_symmetric=True),
+        cm.to_csc_combine(K, M, k0, block_symmetric=True, segment_gather=True)))
     ref_blk = select_csc(ex["indptr"], ex["indices"], cm.to_csc_combine(K, M, k0), sel)[2]
     sub.to_csc_combine(K, M, k0, out=blk)
     torch.cuda.synchronize()
@@ -275,7 +285,10 @@ def suite_csc_pec(args) -> dict:
            "per_frequency_ms": {"to_csc_combine_full": ms_full, "select_block_from_full": ms_sel,
                                 "restricted_to_csc_combine": ms_sub,
                                 "to_csc_combine_full_mirrored": ms_full_m,
-                                "restricted_to_csc_combine_mirrored": ms_sub_m},
+                                "restricted_to_csc_combine_mirrored": ms_sub_m,
+                                "to_csc_combine_full_segment_gather": ms_full_seg,
+                                "to_csc_combine_full_segment_gather_mirrored": ms_full_seg_m},
+           "incidence_equals_segment_gather": same_inc,
            "restricted_equals_selected_full": same,
            "mirrored_equals_plain": same_m,
            "cpu_scipy_select_ms": cpu_ms,
@@ -284,6 +297,9 @@ def suite_csc_pec(args) -> dict:
            "restricted_gather_gb_per_s": (n_contrib * 24 + sub.nnz * 16) / (ms_sub * 1e-3) / 1e9,
            "full_gather_mirrored_gb_per_s":
                (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full_m * 1e-3) / 1e9,
+           "note": "gb_per_s figures use the segment gather's algorithmic bytes (perm 8 + "
+                   "E, B 8 + 8 per contributor, seg 8 + out 8 per entry); the incidence "
+                   "gather moves 2 (pos) + 16 per contributor and 8 per entry",
            "restricted_gather_mirrored_gb_per_s":
                (n_contrib * 24 + sub.nnz * 16) / (ms_sub_m * 1e-3) / 1e9}
     del full, blk, K, M, cm, sub

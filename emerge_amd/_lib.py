@@ -26,6 +26,7 @@ EM_MAT_FULL = 2
 EM_MAT_IS_INVERSE = 1
 
 EM_CSC_BLOCK_SYMMETRIC = 1
+EM_CSC_SEGMENT_GATHER = 2
 
 EM_PREC_FP64 = 0
 EM_PREC_MIXED = 1

@@ -358,12 +358,13 @@ class CscMap:
         return CscMap(None, _handle=h, _device=self.device)
 
     def to_csc(self, data: torch.Tensor, out=None, stream=None,
-               block_symmetric: bool = False) -> torch.Tensor:
+               block_symmetric: bool = False, segment_gather: bool = False) -> torch.Tensor:
         """CSC values of the COO values ``data`` (complex128 or complex64 on device).
 
         ``block_symmetric``: every 20x20 block of ``data`` is symmetric (the assembly with
         scalar, diagonal or symmetric tensors); selects the mirrored-read gather, same
-        result, faster."""
+        result, faster. ``segment_gather`` selects the per-entry perm/seg gather instead of
+        the per-column incidence gather (same result)."""
         prec = PREC_FP64 if data.dtype == torch.complex128 else PREC_MIXED
         if data.dtype not in (torch.complex128, torch.complex64) or data.numel() != self.nnz_coo:
             raise ValueError(f"COO data must be complex with {self.nnz_coo} entries")
@@ -371,14 +372,16 @@ class CscMap:
         if out is None:
             out = torch.empty(self.nnz, dtype=data.dtype, device=self.device)
         _check_out(out, self.nnz, data.dtype)
-        flags = _lib.EM_CSC_BLOCK_SYMMETRIC if block_symmetric else 0
+        flags = ((_lib.EM_CSC_BLOCK_SYMMETRIC if block_symmetric else 0)
+                 | (_lib.EM_CSC_SEGMENT_GATHER if segment_gather else 0))
         check(_lib.lib().em_csc_to_csc_ex(self._h, ctypes.c_void_p(data.data_ptr()), None, 0.0,
                                           ctypes.c_void_p(out.data_ptr()), prec, flags,
                                           _stream_ptr(stream)))
         return out
 
     def to_csc_combine(self, dataE: torch.Tensor, dataB: torch.Tensor, k0: float, out=None,
-                       stream=None, block_symmetric: bool = False) -> torch.Tensor:
+                       stream=None, block_symmetric: bool = False,
+                       segment_gather: bool = False) -> torch.Tensor:
         """CSC values of ``dataE - k0**2 * dataB`` in one pass (assembler.py:494)."""
         if dataE.dtype != dataB.dtype or dataE.numel() != dataB.numel():
             raise ValueError("dataE and dataB must match")
@@ -389,7 +392,8 @@ class CscMap:
         if out is None:
             out = torch.empty(self.nnz, dtype=dataE.dtype, device=self.device)
         _check_out(out, self.nnz, dataE.dtype)
-        flags = _lib.EM_CSC_BLOCK_SYMMETRIC if block_symmetric else 0
+        flags = ((_lib.EM_CSC_BLOCK_SYMMETRIC if block_symmetric else 0)
+                 | (_lib.EM_CSC_SEGMENT_GATHER if segment_gather else 0))
         check(_lib.lib().em_csc_to_csc_ex(self._h, ctypes.c_void_p(dataE.data_ptr()),
                                           ctypes.c_void_p(dataB.data_ptr()), float(k0),
                                           ctypes.c_void_p(out.data_ptr()), prec, flags,

@@ -582,6 +582,8 @@ static void free_csc(em_csc_map_s* c) {
     (void)hipFree(c->b.perm);
     (void)hipFree(c->b.seg);
     (void)hipFree(c->b.inc_ptr);
+    (void)hipFree(c->b.inc);
+    (void)hipFree(c->b.pos);
     delete c;
 }
 
@@ -659,7 +661,7 @@ int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64
 
 static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, void* out,
                      int32_t precision, int32_t flags, void* stream) {
-    if (flags & ~EM_CSC_BLOCK_SYMMETRIC) {
+    if (flags & ~(EM_CSC_BLOCK_SYMMETRIC | EM_CSC_SEGMENT_GATHER)) {
         set_error("unknown to_csc flags");
         return EM_EINVAL;
     }
@@ -676,9 +678,14 @@ static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, 
         return EM_EINVAL;
     }
     DeviceGuard g(map->device);
-    EM_HIP(em::csc_gather(map->b.seg, map->b.perm, map->b.nnz, E, B, k0 * k0, out,
-                          precision == EM_PREC_MIXED ? 1 : 0,
-                          (flags & EM_CSC_BLOCK_SYMMETRIC) ? 1 : 0, (hipStream_t)stream));
+    const int c64 = precision == EM_PREC_MIXED ? 1 : 0;
+    const int mirror = (flags & EM_CSC_BLOCK_SYMMETRIC) ? 1 : 0;
+    if (map->b.inc && map->b.pos && !(flags & EM_CSC_SEGMENT_GATHER))
+        EM_HIP(em::csc_gather_inc(map->b, map->N, E, B, k0 * k0, out, c64, mirror,
+                                  (hipStream_t)stream));
+    else
+        EM_HIP(em::csc_gather(map->b.seg, map->b.perm, map->b.nnz, E, B, k0 * k0, out, c64,
+                              mirror, (hipStream_t)stream));
     return EM_OK;
 }
 

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
     const int* __restrict__ t2f, const int* __restrict__ tetids,
     const long long* __restrict__ inc_ptr, const int* __restrict__ inc, long long N,
     long long* __restrict__ ucount, const long long* __restrict__ indptr,
-    long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg) {
+    long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg,
+    unsigned short* __restrict__ pos) {
     __shared__ unsigned long long key[kColCap];
     __shared__ unsigned long long srt[kColCap];
     const int lane = threadIdx.x;
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
                 const long long ia = e / 20;
                 const int jj = e % 20, ii = li % 20;
                 perm[20 * s + i] = 400 * ia + 20 * ii + jj;
+                pos[20 * s + li] = (unsigned short)(q - indptr[c]);
                 if (flag) {
                     indices[q] = (long long)(cur >> 32);
                     seg[q] = 20 * s + i;
@@ -226,6 +228,19 @@ __device__ __forceinline__ void store_c<float2>(float2* p, long long q, double2 
     p[q] = make_float2((float)v.x, (float)v.y);
 }
 
+// One COO contribution E[k] - k0sq * B[k] in FP64 (one rounding order for every gather).
+template <typename T>
+__device__ __forceinline__ double2 contrib(const T* __restrict__ E, const T* __restrict__ B,
+                                           double k0sq, long long k) {
+    double2 v = LoadC<T>::get(E, k);
+    if (B) {
+        const double2 b = LoadC<T>::get(B, k);
+        v.x = fma(-k0sq, b.x, v.x);
+        v.y = fma(-k0sq, b.y, v.y);
+    }
+    return v;
+}
+
 // COO index of the mirrored entry (jj, ii) of entry k = 400 ia + 20 ii + jj.
 __device__ __forceinline__ long long mirror_coo(long long k) {
     const long long blk = k / 400, e = k - blk * 400;
@@ -250,16 +265,82 @@ __global__ __launch_bounds__(256) void csc_gather_kernel(const long long* __rest
         double2 acc = make_double2(0.0, 0.0);
         for (long long j = j0; j < j1; ++j) {
             const long long k = MIRROR ? mirror_coo(perm[j]) : perm[j];
-            double2 v = LoadC<T>::get(E, k);
-            if (B) {
-                const double2 b = LoadC<T>::get(B, k);
-                v.x -= k0sq * b.x;
-                v.y -= k0sq * b.y;
-            }
+            const double2 v = contrib(E, B, k0sq, k);
             acc.x += v.x;
             acc.y += v.y;
         }
         store_c<T>(out, q, acc);
+    }
+}
+
+// to_csc by column incidences. A column's contributors are the 20 rows ii of every
+// incident (tet ia, slot jj). Each wavefront carries three columns, one per group of 20
+// lanes (lane ii of a group handles row slot ii); a group walks its column's incidence
+// list in order, loads the incidence's 20 values (with MIRROR one contiguous 20-entry
+// row of the block) and adds each into the column's FP64 accumulator in LDS at pos[]
+// (the entry index of the row within the column). LDS operations of a wavefront complete
+// in issue order and the 20 lanes of one incidence hit distinct entries, so every entry
+// sums its contributors in incidence order = ascending COO index, as the perm/seg gather
+// does: the result is the same bit for bit. Columns longer than kIncCap entries are done
+// in windows.
+constexpr int kIncCap = 128;
+constexpr int kIncWaves = 4;  // wavefronts per workgroup
+constexpr int kIncCols = 3;   // columns per wavefront
+
+template <typename T, bool MIRROR, bool HAS_B>
+__global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
+    const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
+    const int* __restrict__ inc, const unsigned short* __restrict__ pos, long long N,
+    const T* __restrict__ E, const T* __restrict__ B, double k0sq, T* __restrict__ out) {
+    __shared__ double acc[kIncWaves * kIncCols][kIncCap][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = lane / 20, ii = lane - 20 * grp;
+    if (grp >= kIncCols) return;  // lanes 60..63: no barriers below, nothing to do
+    double (*a)[2] = acc[wave * kIncCols + grp];
+    const long long stride = (long long)gridDim.x * kIncWaves * kIncCols;
+    for (long long c = ((long long)blockIdx.x * kIncWaves + wave) * kIncCols + grp; c < N;
+         c += stride) {
+        const long long cb = indptr[c], len = indptr[c + 1] - cb;
+        const long long s = inc_ptr[c], ninc = inc_ptr[c + 1] - s;
+        for (long long w0 = 0; w0 < len; w0 += kIncCap) {
+            const int wl = (int)(len - w0 < kIncCap ? len - w0 : kIncCap);
+            for (int p = ii; p < wl; p += 20) {
+                a[p][0] = 0.0;
+                a[p][1] = 0.0;
+            }
+            // batches of kU incidences: all loads of a batch are issued before the adds
+            constexpr int kU = 4;
+            for (long long i0 = 0; i0 < ninc; i0 += kU) {
+                int p[kU], e[kU];
+                double2 v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const long long i = i0 + u < ninc ? i0 + u : ninc - 1;  // in range
+                    e[u] = inc[s + i];
+                    p[u] = i0 + u < ninc ? (int)pos[20 * (s + i) + ii] - (int)w0 : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const long long ia = e[u] / 20;
+                    const int jj = e[u] - 20 * (int)ia;
+                    const long long k = 400 * ia + (MIRROR ? 20 * jj + ii : 20 * ii + jj);
+                    v[u] = LoadC<T>::get(E, k);
+                    if (HAS_B) {
+                        const double2 b = LoadC<T>::get(B, k);
+                        v[u].x = fma(-k0sq, b.x, v[u].x);
+                        v[u].y = fma(-k0sq, b.y, v[u].y);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (p[u] >= 0 && p[u] < wl) {
+                        atomicAdd(&a[p[u]][0], v[u].x);
+                        atomicAdd(&a[p[u]][1], v[u].y);
+                    }
+            }
+            for (int p = ii; p < wl; p += 20) store_c<T>(out, cb + w0 + p,
+                                                         make_double2(a[p][0], a[p][1]));
+        }
     }
 }
 
@@ -302,6 +383,8 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
     TRY(hipMalloc((void**)&b.inc_ptr, sizeof(long long) * (N + 1)));
     TRY(hipMalloc((void**)&maxlen, sizeof(unsigned long long)));
     TRY(hipMalloc((void**)&inc, sizeof(int) * (size_t)(m.n_assy * 20 > 0 ? m.n_assy * 20 : 1)));
+    TRY(hipMalloc((void**)&b.pos,
+                  sizeof(unsigned short) * (size_t)(m.n_assy * 400 > 0 ? m.n_assy * 400 : 1)));
     TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
     TRY(hipMemsetAsync(maxlen, 0, sizeof(unsigned long long), s));
     if (m.n_assy > 0) {
@@ -336,7 +419,8 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
         const unsigned g = (unsigned)(N < (1 << 20) ? N : (1 << 20));
         if (N > 0) {
             hipLaunchKernelGGL(csc_column_kernel<false>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
-                               b.inc_ptr, inc, N, ucount, nullptr, nullptr, nullptr, nullptr);
+                               b.inc_ptr, inc, N, ucount, nullptr, nullptr, nullptr, nullptr,
+                               nullptr);
             TRY(hipGetLastError());
         }
         TRY(hipMalloc((void**)&b.indptr, sizeof(long long) * (N + 1)));
@@ -349,14 +433,15 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
         TRY(hipMemcpyAsync(b.seg + b.nnz, &nnz_coo, sizeof(long long), hipMemcpyHostToDevice, s));
         if (N > 0) {
             hipLaunchKernelGGL(csc_column_kernel<true>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
-                               b.inc_ptr, inc, N, nullptr, b.indptr, b.indices, b.perm, b.seg);
+                               b.inc_ptr, inc, N, nullptr, b.indptr, b.indices, b.perm, b.seg,
+                               b.pos);
             TRY(hipGetLastError());
         }
         TRY(hipStreamSynchronize(s));
     }
     (void)hipFree(cnt);
     (void)hipFree(maxlen);
-    (void)hipFree(inc);
+    b.inc = inc;  // kept for the incidence gather
     return hipSuccess;
 fail:
     (void)hipStreamSynchronize(s);
@@ -368,6 +453,7 @@ fail:
     (void)hipFree(b.perm);
     (void)hipFree(b.seg);
     (void)hipFree(b.inc_ptr);
+    (void)hipFree(b.pos);
     b = CscBuild();
     return e;
 #undef TRY
@@ -392,6 +478,37 @@ hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz
         launch_gather<float2>(mirror != 0, seg, perm, nnz, E, B, k0sq, out, s);
     else
         launch_gather<double2>(mirror != 0, seg, perm, nnz, E, B, k0sq, out, s);
+    return hipGetLastError();
+}
+
+template <typename T, bool MIRROR>
+static void launch_inc(const CscBuild& b, long long N, const void* E, const void* B, double k0sq,
+                       void* out, hipStream_t s) {
+    const long long per_block = kIncWaves * kIncCols;
+    const long long nb = (N + per_block - 1) / per_block;
+    const dim3 g((unsigned)(nb < (1 << 20) ? nb : (1 << 20))), blk(64 * kIncWaves);
+    if (B)
+        hipLaunchKernelGGL((csc_gather_inc_kernel<T, MIRROR, true>), g, blk, 0, s, b.indptr,
+                           b.inc_ptr, b.inc, b.pos, N, (const T*)E, (const T*)B, k0sq, (T*)out);
+    else
+        hipLaunchKernelGGL((csc_gather_inc_kernel<T, MIRROR, false>), g, blk, 0, s, b.indptr,
+                           b.inc_ptr, b.inc, b.pos, N, (const T*)E, (const T*)B, k0sq, (T*)out);
+}
+
+hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,
+                          double k0sq, void* out, int complex64, int mirror, hipStream_t s) {
+    if (N == 0 || b.nnz == 0) return hipSuccess;
+    if (complex64) {
+        if (mirror)
+            launch_inc<float2, true>(b, N, E, B, k0sq, out, s);
+        else
+            launch_inc<float2, false>(b, N, E, B, k0sq, out, s);
+    } else {
+        if (mirror)
+            launch_inc<double2, true>(b, N, E, B, k0sq, out, s);
+        else
+            launch_inc<double2, false>(b, N, E, B, k0sq, out, s);
+    }
     return hipGetLastError();
 }
 

@@ -75,6 +75,8 @@ struct CscBuild {
     long long* perm = nullptr;     // [nnz_coo]
     long long* seg = nullptr;      // [nnz+1]
     long long* inc_ptr = nullptr;  // [N+1]
+    int* inc = nullptr;             // [20 Ta] incidences (20 ia + slot) of each DOF, ascending
+    unsigned short* pos = nullptr;  // [400 Ta] entry index of row ii within the column
     long long nnz = 0;
 };
 hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
@@ -85,6 +87,9 @@ hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz
                       hipStream_t s);
 hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s);
 hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s);
+// to_csc by column incidences (full maps: inc and pos present); same result as csc_gather.
+hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,
+                          double k0sq, void* out, int complex64, int mirror, hipStream_t s);
 // Exclusive scan of n int64 values (in may equal out); allocates temporaries, blocks.
 hipError_t exclusive_scan_i64(const long long* in, long long* out, long long n, hipStream_t s);
 

@@ -181,6 +181,13 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
  * result equal to that level.
  */
 #define EM_CSC_BLOCK_SYMMETRIC 1
+/*
+ * A full map sums each column from its tet incidences (one wavefront per column, values
+ * read row by row from the blocks); EM_CSC_SEGMENT_GATHER selects the per-entry gather
+ * through perm/seg instead (what restricted maps use). Both give the same result bit for
+ * bit; the flag exists to compare them.
+ */
+#define EM_CSC_SEGMENT_GATHER 2
 int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                      void* data_csc, int32_t precision, int32_t flags, void* stream);
 

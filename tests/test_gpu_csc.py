@@ -138,3 +138,48 @@ def test_mirrored_gather_symmetric_full_tensors(delaunay_field):
     assert float(torch.linalg.norm(blk - blk.transpose(1, 2)) / torch.linalg.norm(blk)) < 1e-15
     a, b = cm.to_csc(A), cm.to_csc(A, block_symmetric=True)
     assert float(torch.linalg.norm(a - b) / torch.linalg.norm(a)) < 1e-15
+
+
+def _fan_field(n):
+    """n tets around one edge (bottom-top axis, ring of n nodes): the axis edge's DOF
+    columns hold more than 256 distinct rows for n = 30."""
+    from emerge_amd.mesh import Nedelec2Field, TetMesh, orient_positive
+    ang = 2 * np.pi * np.arange(n) / n
+    nodes = np.concatenate([[[0.0, 0.0, 0.0], [0.0, 0.0, 1.0]],
+                            np.stack([np.cos(ang), np.sin(ang), np.full(n, 0.5)], 1)]).T
+    tets = np.array([[0, 1, 2 + i, 2 + (i + 1) % n] for i in range(n)]).T
+    return Nedelec2Field(TetMesh(nodes, orient_positive(nodes, tets)))
+
+
+@pytest.mark.parametrize("mesh", ["delaunay", "fan30"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_incidence_gather_equals_segment_gather(delaunay_field, mesh, precision):
+    """The per-column incidence gather (default for full maps) and the per-entry perm/seg
+    gather give the same values bit for bit (same contributors, same order), plain and
+    mirrored, to_csc and combine; long columns (> 256 entries) go through LDS windows."""
+    import torch
+    from emerge_amd.assembly import CscMap
+    field = delaunay_field if mesh == "delaunay" else _fan_field(30)
+    rng = np.random.default_rng(24)
+    T = field.n_tets
+    er, ur = ref_tensor(T, "scalar", rng), ref_tensor(T, "diag", rng)
+    plan = _field_plan(field)
+    cm = CscMap(plan)
+    lens = np.diff(cm.export("indptr")["indptr"].cpu().numpy())
+    if mesh == "fan30":
+        assert lens.max() > 256
+    k0 = 2 * np.pi * 8e9 / 299792458.0
+    K, M = plan.assemble_km(er, ur, precision)
+    for sym in (False, True):
+        a = cm.to_csc(K, block_symmetric=sym)
+        b = cm.to_csc(K, block_symmetric=sym, segment_gather=True)
+        assert torch.equal(a, b), sym
+        a = cm.to_csc_combine(K, M, k0, block_symmetric=sym)
+        b = cm.to_csc_combine(K, M, k0, block_symmetric=sym, segment_gather=True)
+        assert torch.equal(a, b), sym
+    # and against the oracle's CSC of the same mesh
+    if precision == "fp64":
+        dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(field, er, ur, np.array([]),
+                                                                local_frame=True)
+        ref = oracle.CSCMap(rows, cols, field.n_field).to_csc_data(dE - k0 ** 2 * dB)
+        assert rel_fro(cm.to_csc_combine(K, M, k0).cpu().numpy(), ref) <= 1e-12

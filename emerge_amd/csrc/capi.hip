@@ -584,6 +584,7 @@ static void free_csc(em_csc_map_s* c) {
     (void)hipFree(c->b.inc_ptr);
     (void)hipFree(c->b.inc);
     (void)hipFree(c->b.pos);
+    (void)hipFree(c->b.cols);
     delete c;
 }
 

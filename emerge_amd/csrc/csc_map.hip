@@ -290,8 +290,9 @@ constexpr int kIncCols = 3;   // columns per wavefront
 template <typename T, bool MIRROR, bool HAS_B>
 __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
     const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
-    const int* __restrict__ inc, const unsigned short* __restrict__ pos, long long N,
-    const T* __restrict__ E, const T* __restrict__ B, double k0sq, T* __restrict__ out) {
+    const int* __restrict__ inc, const unsigned short* __restrict__ pos,
+    const long long* __restrict__ cols, long long N, const T* __restrict__ E,
+    const T* __restrict__ B, double k0sq, T* __restrict__ out) {
     __shared__ double acc[kIncWaves * kIncCols][kIncCap][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane / 20, ii = lane - 20 * grp;
@@ -301,7 +302,8 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
     for (long long c = ((long long)blockIdx.x * kIncWaves + wave) * kIncCols + grp; c < N;
          c += stride) {
         const long long cb = indptr[c], len = indptr[c + 1] - cb;
-        const long long s = inc_ptr[c], ninc = inc_ptr[c + 1] - s;
+        const long long fc = cols ? cols[c] : c;  // column of the incidence lists
+        const long long s = inc_ptr[fc], ninc = inc_ptr[fc + 1] - s;
         for (long long w0 = 0; w0 < len; w0 += kIncCap) {
             const int wl = (int)(len - w0 < kIncCap ? len - w0 : kIncCap);
             for (int p = ii; p < wl; p += 20) {
@@ -489,10 +491,12 @@ static void launch_inc(const CscBuild& b, long long N, const void* E, const void
     const dim3 g((unsigned)(nb < (1 << 20) ? nb : (1 << 20))), blk(64 * kIncWaves);
     if (B)
         hipLaunchKernelGGL((csc_gather_inc_kernel<T, MIRROR, true>), g, blk, 0, s, b.indptr,
-                           b.inc_ptr, b.inc, b.pos, N, (const T*)E, (const T*)B, k0sq, (T*)out);
+                           b.inc_ptr, b.inc, b.pos, b.cols, N, (const T*)E, (const T*)B, k0sq,
+                           (T*)out);
     else
         hipLaunchKernelGGL((csc_gather_inc_kernel<T, MIRROR, false>), g, blk, 0, s, b.indptr,
-                           b.inc_ptr, b.inc, b.pos, N, (const T*)E, (const T*)B, k0sq, (T*)out);
+                           b.inc_ptr, b.inc, b.pos, b.cols, N, (const T*)E, (const T*)B, k0sq,
+                           (T*)out);
 }
 
 hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,

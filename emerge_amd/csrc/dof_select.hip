@@ -200,7 +200,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void restrict_fill_kernel(
     const long long* __restrict__ seg, const long long* __restrict__ perm,
     const long long* __restrict__ new_index, const long long* __restrict__ sub_indptr,
     const long long* __restrict__ sub_colperm, long long* __restrict__ sub_indices,
-    long long* __restrict__ sub_seg, long long* __restrict__ sub_perm) {
+    long long* __restrict__ sub_seg, long long* __restrict__ sub_perm,
+    long long* __restrict__ newq) {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (long long c = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); c < n;
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void restrict_fill_kernel(
                 if (lane >= off) inc += t;
             }
             const unsigned long long m = __ballot(k);
+            if (j < indptr[c + 1]) newq[j] = k ? out + __popcll(m & below) : -1;
             if (k) {
                 const long long p = out + __popcll(m & below);
                 const long long ps = pout + inc - len;
@@ -230,6 +232,28 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void restrict_fill_kernel(
             }
             out += __popcll(m);
             pout += __shfl(inc, 63);
+        }
+    }
+}
+
+// Restricted map, pass 3 (incidence gather tables): for every kept column, the entry
+// index of each incidence row within the block's column (0xffff: row removed), and the
+// full-matrix column of every block column.
+__global__ __launch_bounds__(64 * kWavesPerBlock) void restrict_pos_kernel(
+    long long n, const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
+    const unsigned short* __restrict__ pos, const long long* __restrict__ new_index,
+    const long long* __restrict__ newq, const long long* __restrict__ sub_indptr,
+    unsigned short* __restrict__ sub_pos, long long* __restrict__ cols) {
+    const int lane = threadIdx.x & 63;
+    for (long long c = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); c < n;
+         c += (long long)gridDim.x * kWavesPerBlock) {
+        const long long nc = new_index[c];
+        if (nc < 0) continue;
+        if (lane == 0) cols[nc] = c;
+        const long long x0 = 20 * inc_ptr[c], x1 = 20 * inc_ptr[c + 1];
+        for (long long x = x0 + lane; x < x1; x += 64) {
+            const long long nq = newq[indptr[c] + pos[x]];
+            sub_pos[x] = nq >= 0 ? (unsigned short)(nq - sub_indptr[nc]) : (unsigned short)0xffff;
         }
     }
 }
@@ -372,6 +396,9 @@ hipError_t csc_select_fill(long long n, const long long* indptr, const long long
 hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_index,
                         long long n_solve, CscBuild& sub, long long& n_perm, hipStream_t s) {
     long long* colperm = nullptr;
+    long long* newq = nullptr;
+    const bool inc_tables = full.inc && full.pos;
+    long long n_inc = 0;
     hipError_t e;
 #define TRY(x)                          \
     do {                                \
@@ -397,22 +424,51 @@ hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_
     TRY(hipMalloc((void**)&sub.seg, sizeof(long long) * (sub.nnz + 1)));
     TRY(hipMalloc((void**)&sub.perm, sizeof(long long) * (n_perm > 0 ? n_perm : 1)));
     TRY(hipMemcpyAsync(sub.seg + sub.nnz, &n_perm, sizeof(long long), hipMemcpyHostToDevice, s));
+    TRY(hipMalloc((void**)&newq, sizeof(long long) * (full.nnz > 0 ? full.nnz : 1)));
     if (n > 0) {
         hipLaunchKernelGGL(restrict_fill_kernel, dim3(col_grid(n)), dim3(64 * kWavesPerBlock), 0,
                            s, n, full.indptr, full.indices, full.seg, full.perm, new_index,
-                           sub.indptr, colperm, sub.indices, sub.seg, sub.perm);
+                           sub.indptr, colperm, sub.indices, sub.seg, sub.perm, newq);
         TRY(hipGetLastError());
+    }
+    if (inc_tables) {
+        // incidence-gather tables: the full map's incidence lists, row positions within the
+        // block's columns, and the full column of each block column
+        TRY(hipMemcpyAsync(&n_inc, full.inc_ptr + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+        TRY(hipStreamSynchronize(s));
+        TRY(hipMalloc((void**)&sub.inc_ptr, sizeof(long long) * (n + 1)));
+        TRY(hipMalloc((void**)&sub.inc, sizeof(int) * (n_inc > 0 ? n_inc : 1)));
+        TRY(hipMalloc((void**)&sub.pos, sizeof(unsigned short) * (n_inc > 0 ? 20 * n_inc : 1)));
+        TRY(hipMalloc((void**)&sub.cols, sizeof(long long) * (n_solve > 0 ? n_solve : 1)));
+        TRY(hipMemcpyAsync(sub.inc_ptr, full.inc_ptr, sizeof(long long) * (n + 1),
+                           hipMemcpyDeviceToDevice, s));
+        if (n_inc > 0) {
+            TRY(hipMemcpyAsync(sub.inc, full.inc, sizeof(int) * n_inc, hipMemcpyDeviceToDevice, s));
+            TRY(hipMemsetAsync(sub.pos, 0xff, sizeof(unsigned short) * 20 * n_inc, s));
+        }
+        if (n > 0) {
+            hipLaunchKernelGGL(restrict_pos_kernel, dim3(col_grid(n)), dim3(64 * kWavesPerBlock),
+                               0, s, n, full.indptr, full.inc_ptr, full.pos, new_index, newq,
+                               sub.indptr, sub.pos, sub.cols);
+            TRY(hipGetLastError());
+        }
     }
     TRY(hipStreamSynchronize(s));
     (void)hipFree(colperm);
+    (void)hipFree(newq);
     return hipSuccess;
 fail:
     (void)hipStreamSynchronize(s);
     (void)hipFree(colperm);
+    (void)hipFree(newq);
     (void)hipFree(sub.indptr);
     (void)hipFree(sub.indices);
     (void)hipFree(sub.seg);
     (void)hipFree(sub.perm);
+    (void)hipFree(sub.inc_ptr);
+    (void)hipFree(sub.inc);
+    (void)hipFree(sub.pos);
+    (void)hipFree(sub.cols);
     sub = CscBuild();
     return e;
 #undef TRY

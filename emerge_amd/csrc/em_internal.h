@@ -77,6 +77,7 @@ struct CscBuild {
     long long* inc_ptr = nullptr;  // [N+1]
     int* inc = nullptr;             // [20 Ta] incidences (20 ia + slot) of each DOF, ascending
     unsigned short* pos = nullptr;  // [400 Ta] entry index of row ii within the column
+    long long* cols = nullptr;      // restricted maps: full-matrix column of each column
     long long nnz = 0;
 };
 hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
@@ -110,8 +111,9 @@ hipError_t csc_select_fill(long long n, const long long* indptr, const long long
                            const void* data, int complex64, int drop_zeros,
                            const long long* new_index, const long long* sel_indptr,
                            long long* sel_indices, void* sel_data, hipStream_t s);
-// Map of the kept rows/columns of `full` (N columns). sub.inc_ptr stays null; n_perm is
-// the number of COO contributors kept.
+// Map of the kept rows/columns of `full` (N columns): perm/seg of the kept contributors
+// (n_perm of them) and, when `full` has them, incidence-gather tables (inc_ptr/inc of
+// the full columns, pos with 0xffff for removed rows, cols).
 hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_index,
                         long long n_solve, CscBuild& sub, long long& n_perm, hipStream_t s);
 hipError_t dof_move(const long long* ids, long long n_ids, long long ncols, const void* src,

@@ -146,6 +146,10 @@ def test_restricted_map_equals_selected_full(delaunay_field, precision):
     exs = sub.export("indptr", "indices")
     assert torch.equal(exs["indptr"], sp_) and torch.equal(exs["indices"], si)
     assert torch.equal(blk, sd)
+    # the restricted map's incidence gather = its per-entry segment gather, plain and mirrored
+    assert torch.equal(blk, sub.to_csc_combine(K, M, k0, segment_gather=True))
+    assert torch.equal(sub.to_csc_combine(K, M, k0, block_symmetric=True),
+                       sub.to_csc_combine(K, M, k0, block_symmetric=True, segment_gather=True))
     one = sub.to_csc(K)
     assert torch.equal(one, select_csc(ex["indptr"], ex["indices"], cm.to_csc(K), sel)[2])
     # against the oracle, assembled and sliced on the CPU

@@ -28,6 +28,9 @@ EM_MAT_IS_INVERSE = 1
 EM_CSC_BLOCK_SYMMETRIC = 1
 EM_CSC_SEGMENT_GATHER = 2
 
+EM_TRI_BY_FACE = 0
+EM_TRI_COMPACT = 1
+
 EM_PREC_FP64 = 0
 EM_PREC_MIXED = 1
 
@@ -95,6 +98,12 @@ _SIGS = {
     "em_csc_map_restrict": (ctypes.c_int, [P, P, I64, ctypes.POINTER(P), P]),
     "em_dof_gather": (ctypes.c_int, [P, I64, I64, P, P, I32, P]),
     "em_dof_scatter": (ctypes.c_int, [P, I64, I64, P, P, I32, P]),
+    "em_ned2_robin_blocks": (ctypes.c_int, [P, P, I64, P, I64, ctypes.c_double, ctypes.c_double,
+                                            P, I32, P]),
+    "em_ned2_robin_force": (ctypes.c_int, [P, P, I64, P, P, I64, P, P, P, P, P]),
+    "em_tri_rowcol": (ctypes.c_int, [P, I64, P, I64, P, P, P]),
+    "em_csc_map_find": (ctypes.c_int, [P, P, P, I64, P, P, P]),
+    "em_csc_add": (ctypes.c_int, [P, I32, P, P, I64, ctypes.c_double, ctypes.c_double, P]),
     "em_graph_begin_capture": (ctypes.c_int, [P]),
     "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "em_graph_launch": (ctypes.c_int, [P, P]),

@@ -850,4 +850,93 @@ int em_dof_scatter(const int64_t* ids, int64_t n_ids, int64_t ncols, const void*
     return dof_move(ids, n_ids, ncols, src, dst, precision, 1, stream);
 }
 
+// ---------------------------------------------------------------------------------------
+// Robin / port surface assembly
+// ---------------------------------------------------------------------------------------
+
+int em_ned2_robin_blocks(em_ned2_plan_t plan, const int64_t* tris, int64_t n_tris,
+                         const int64_t* surf, int64_t n_surf, double gamma_re, double gamma_im,
+                         void* out, int32_t layout, void* stream) {
+    if (!plan || n_surf < 0 || n_tris < 0 || (n_surf && (!tris || !surf || !out)) ||
+        (layout != EM_TRI_BY_FACE && layout != EM_TRI_COMPACT)) {
+        set_error("robin_blocks: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    long long bad = 0;
+    EM_HIP(em::robin_blocks(plan->xyz, plan->n_nodes, (const long long*)tris, n_tris,
+                            (const long long*)surf, n_surf, gamma_re, gamma_im, (double2*)out,
+                            layout == EM_TRI_BY_FACE, bad, (hipStream_t)stream));
+    if (bad) {
+        set_error("robin_blocks: face or node id out of range");
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_ned2_robin_force(em_ned2_plan_t plan, const int64_t* tris, int64_t n_tris,
+                        const int64_t* tri_to_field, const int64_t* surf, int64_t n_surf,
+                        const void* U, const void* U_curl, const double* normals, void* bvec,
+                        void* stream) {
+    if (!plan || n_surf < 0 || n_tris < 0 ||
+        (n_surf && (!tris || !tri_to_field || !surf || !U || !bvec)) ||
+        (U_curl && !normals)) {
+        set_error("robin_force: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    long long bad = 0;
+    EM_HIP(em::robin_force(plan->xyz, plan->n_nodes, (const long long*)tris, n_tris,
+                           (const long long*)tri_to_field, plan->n_field, (const long long*)surf,
+                           n_surf, (const double2*)U, (const double2*)U_curl, normals,
+                           (double2*)bvec, bad, (hipStream_t)stream));
+    if (bad) {
+        set_error((bad & 1) ? "robin_force: face or node id out of range"
+                            : "robin_force: tri_to_field entry out of range");
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_tri_rowcol(const int64_t* tri_to_field, int64_t n_tris, const int64_t* surf,
+                  int64_t n_surf, int64_t* rows, int64_t* cols, void* stream) {
+    if (n_surf < 0 || n_tris < 0 || (n_surf && (!tri_to_field || !surf || !rows || !cols))) {
+        set_error("tri_rowcol: bad argument");
+        return EM_EINVAL;
+    }
+    long long bad = 0;
+    EM_HIP(em::tri_rowcol((const long long*)tri_to_field, n_tris, (const long long*)surf, n_surf,
+                          (long long*)rows, (long long*)cols, bad, (hipStream_t)stream));
+    if (bad) {
+        set_error("tri_rowcol: face id out of range");
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_csc_map_find(em_csc_map_t map, const int64_t* rows, const int64_t* cols, int64_t n,
+                    const int64_t* new_index, int64_t* pos, void* stream) {
+    if (!map || n < 0 || (n && (!rows || !cols || !pos))) {
+        set_error("csc_map_find: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(map->device);
+    EM_HIP(em::csc_find(map->b.indptr, map->b.indices, map->N, (const long long*)rows,
+                        (const long long*)cols, (const long long*)new_index, n, (long long*)pos,
+                        (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_csc_add(void* data_csc, int32_t precision, const int64_t* pos, const void* values,
+               int64_t n, double scale_re, double scale_im, void* stream) {
+    if (int st = check_prec(precision)) return st;
+    if (n < 0 || (n && (!data_csc || !pos || !values))) {
+        set_error("csc_add: bad argument");
+        return EM_EINVAL;
+    }
+    EM_HIP(em::csc_add(data_csc, precision == EM_PREC_MIXED, (const long long*)pos,
+                       (const double2*)values, n, scale_re, scale_im, (hipStream_t)stream));
+    return EM_OK;
+}
+
 }  // extern "C"

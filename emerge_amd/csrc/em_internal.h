@@ -119,6 +119,24 @@ hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_
 hipError_t dof_move(const long long* ids, long long n_ids, long long ncols, const void* src,
                     void* dst, int complex64, int scatter, hipStream_t s);
 
+// Robin / port surface assembly (robin.hip). `bad` bits: 1 face or node id out of range,
+// 2 DOF out of range. The first three block until done.
+hipError_t robin_blocks(const double4* xyz, long long n_nodes, const long long* tris, long long F,
+                        const long long* surf, long long nsurf, double gre, double gim,
+                        double2* out, int by_face, long long& bad, hipStream_t s);
+hipError_t robin_force(const double4* xyz, long long n_nodes, const long long* tris, long long F,
+                       const long long* tri_to_field, long long n_field, const long long* surf,
+                       long long nsurf, const double2* U, const double2* Uc, const double* normals,
+                       double2* bvec, long long& bad, hipStream_t s);
+hipError_t tri_rowcol(const long long* tri_to_field, long long F, const long long* surf,
+                      long long nsurf, long long* rows, long long* cols, long long& bad,
+                      hipStream_t s);
+hipError_t csc_find(const long long* indptr, const long long* indices, long long N,
+                    const long long* rows, const long long* cols, const long long* new_index,
+                    long long n, long long* pos, hipStream_t s);
+hipError_t csc_add(void* data, int complex64, const long long* pos, const double2* vals,
+                   long long n, double sre, double sim, hipStream_t s);
+
 void set_error(const std::string& msg);
 
 }  // namespace em

@@ -247,6 +247,51 @@ int em_dof_scatter(const int64_t* ids, int64_t n_ids, int64_t ncols, const void*
                    int32_t precision, void* stream);
 
 /*
+ * Robin / port surface assembly on boundary triangles (physics/microwave/assembly/
+ * robinbc.py). The plan supplies the node coordinates; tris is the mesh's face table
+ * (int64 [3][n_tris], sorted node ids per face, mesh3d.py:707-732) and tri_to_field the
+ * field's face DOF table (int64 [8][n_tris], nedelec2.py:70-79), both DEVICE pointers,
+ * as are surf (face ids) and every array below. The first three block (they check ids).
+ *
+ * em_ned2_robin_blocks: ned2_tri_stiff + compute_bc_entries (robinbc.py:316-402):
+ * the 8x8 block gamma * int f_i . f_j dA of every listed face (row-major, complex128),
+ * ADDED to out at face * 64 (EM_TRI_BY_FACE: out is the reference's Bmat of n_tris * 64)
+ * or at list position * 64 (EM_TRI_COMPACT: n_surf * 64). A face listed twice in one call
+ * races, as in the reference's prange loop.
+ */
+#define EM_TRI_BY_FACE 0
+#define EM_TRI_COMPACT 1
+int em_ned2_robin_blocks(em_ned2_plan_t plan, const int64_t* tris, int64_t n_tris,
+                         const int64_t* surf, int64_t n_surf, double gamma_re, double gamma_im,
+                         void* out, int32_t layout, void* stream);
+/*
+ * ned2_tri_force + compute_force_entries (robinbc.py:220-313): bvec[tri_to_field[i, f]] +=
+ * -A sum_w w_w f_i(x_w) . U(x_w) with the reference's 6-point rule; U is the caller's mode
+ * field at the rule's points, complex128 [3][6][n_surf] (U_global_all, the points of
+ * generate_points_3d, robinbc.py:111-135). With U_curl (same layout) and normals (double
+ * [3][n_surf]) it is the scattered-field form ned2_tri_force_scat (robinbc.py:453-569).
+ * bvec: complex128 [n_field], accumulated atomically.
+ */
+int em_ned2_robin_force(em_ned2_plan_t plan, const int64_t* tris, int64_t n_tris,
+                        const int64_t* tri_to_field, const int64_t* surf, int64_t n_surf,
+                        const void* U, const void* U_curl, const double* normals, void* bvec,
+                        void* stream);
+/* rows/cols (int64 [n_surf * 64]) of the listed faces' blocks (femdata.py:75-94). */
+int em_tri_rowcol(const int64_t* tri_to_field, int64_t n_tris, const int64_t* surf,
+                  int64_t n_surf, int64_t* rows, int64_t* cols, void* stream);
+/*
+ * K += generate_csc(B) (assembler.py:635; femdata.py:104-119) on the device: the CSC
+ * position of each (rows[i], cols[i]) in the map's pattern, -1 if absent (new_index, when
+ * not NULL, renumbers rows/cols first: for a restricted map, removed DOFs give -1); then
+ * data_csc[pos[i]] += scale * values[i] (values complex128; data complex128 or complex64
+ * per precision; atomic, entries of pos -1 skipped).
+ */
+int em_csc_map_find(em_csc_map_t map, const int64_t* rows, const int64_t* cols, int64_t n,
+                    const int64_t* new_index, int64_t* pos, void* stream);
+int em_csc_add(void* data_csc, int32_t precision, const int64_t* pos, const void* values,
+               int64_t n, double scale_re, double scale_im, void* stream);
+
+/*
  * HIP graphs of assembly launches. The reference re-runs its Numba builder per call
  * (microwave_3d.py:1457 sweep loop) and has no counterpart: these record the launches
  * issued on `stream` between begin and end (hipStreamBeginCapture, thread-local mode)

@@ -31,6 +31,9 @@ EM_CSC_SEGMENT_GATHER = 2
 EM_TRI_BY_FACE = 0
 EM_TRI_COMPACT = 1
 
+EM_INTERP_E = 0
+EM_INTERP_CURL = 1
+
 EM_PREC_FP64 = 0
 EM_PREC_MIXED = 1
 
@@ -104,6 +107,8 @@ _SIGS = {
     "em_tri_rowcol": (ctypes.c_int, [P, I64, P, I64, P, P, P]),
     "em_csc_map_find": (ctypes.c_int, [P, P, P, I64, P, P, P]),
     "em_csc_add": (ctypes.c_int, [P, I32, P, P, I64, ctypes.c_double, ctypes.c_double, P]),
+    "em_ned2_locate": (ctypes.c_int, [P, P, I64, P, I64, P, P]),
+    "em_ned2_interp": (ctypes.c_int, [P, P, I64, P, P, I32, P, P, P]),
     "em_graph_begin_capture": (ctypes.c_int, [P]),
     "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "em_graph_launch": (ctypes.c_int, [P, P]),

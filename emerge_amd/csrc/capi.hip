@@ -939,4 +939,43 @@ int em_csc_add(void* data_csc, int32_t precision, const int64_t* pos, const void
     return EM_OK;
 }
 
+// ---------------------------------------------------------------------------------------
+// Point location and field interpolation
+// ---------------------------------------------------------------------------------------
+
+int em_ned2_locate(em_ned2_plan_t plan, const double* coords, int64_t n, const int64_t* tetids,
+                   int64_t n_tetids, int64_t* tet_of, void* stream) {
+    if (!plan || n < 0 || n_tetids < 0 || (n && (!coords || !tet_of))) {
+        set_error("locate: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    long long bad = 0;
+    const long long n_cand = tetids ? n_tetids : plan->n_tets;
+    EM_HIP(em::locate_points(plan->xyz, plan->tets, plan->n_tets, coords, n,
+                             (const long long*)tetids, n_cand, (long long*)tet_of, bad,
+                             (hipStream_t)stream));
+    if (bad) {
+        set_error("locate: candidate tet id out of range");
+        return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+int em_ned2_interp(em_ned2_plan_t plan, const double* coords, int64_t n, const int64_t* tet_of,
+                   const void* solution, int32_t which, const void* c_tet, void* out,
+                   void* stream) {
+    if (!plan || n < 0 || (n && (!coords || !tet_of || !solution || !out)) ||
+        (which != EM_INTERP_E && which != EM_INTERP_CURL)) {
+        set_error("interp: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    EM_HIP(em::ned2_interp(plan->xyz, plan->tets, plan->t2f, plan->n_tets, coords, n,
+                           (const long long*)tet_of, (const double2*)solution,
+                           which == EM_INTERP_CURL, (const double2*)c_tet, (double2*)out,
+                           (hipStream_t)stream));
+    return EM_OK;
+}
+
 }  // extern "C"

@@ -137,6 +137,16 @@ hipError_t csc_find(const long long* indptr, const long long* indices, long long
 hipError_t csc_add(void* data, int complex64, const long long* pos, const double2* vals,
                    long long n, double sre, double sim, hipStream_t s);
 
+// Point location and field interpolation (interp.hip). locate_points: out[j] = the largest
+// candidate tet id containing point j (coords [3][n]), -1 if none; bad bit 1: candidate id
+// out of range. Blocks.
+hipError_t locate_points(const double4* xyz, const int4* tets, long long n_tets, const double* c,
+                         long long n, const long long* tetids, long long n_cand,
+                         long long* out, long long& bad, hipStream_t s);
+hipError_t ned2_interp(const double4* xyz, const int4* tets, const int* t2f, long long n_tets,
+                       const double* c, long long n, const long long* tet_of, const double2* sol,
+                       int curl, const double2* ctet, double2* out, hipStream_t s);
+
 void set_error(const std::string& msg);
 
 }  // namespace em

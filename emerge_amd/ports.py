@@ -14,8 +14,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+# const.py:20-24
 C0 = 299792458.0
-MU0 = 1.25663706212e-6  # port_bcs.py uses this value for omega mu
+Z0 = 376.73031366857
+EPS0 = 8.854187818814e-12
+MU0 = 1 / (C0 * C0 * EPS0)
 
 
 @dataclass
@@ -87,11 +90,13 @@ class RectangularWaveguideMode:
         Eh = amp * np.sin(np.pi * m * xl / self.width) * np.sin(np.pi * n * yl / self.height)
         return np.array([Eh, Ev, 0 * Eh])
 
-    def port_mode_3d_global(self, x, y, z, k0: float):
-        """port_bcs.py:1064-1077."""
+    def port_mode_3d_global(self, x, y, z, k0: float, mode_nr=None, which: str = "E"):
+        """port_bcs.py:1064-1077 (which "E" or "Exy"; the TE field has no local z part)."""
+        if which not in ("E", "Exy"):
+            raise ValueError(f"which must be 'E' or 'Exy', not {which}")
         xl, yl, _ = self.frame.in_local_cs(x, y, z)
         Ex, Ey, Ez = self.port_mode_3d(xl, yl, k0)
-        return np.array(self.frame.in_global_basis(Ex, Ey, Ez))
+        return np.array(self.frame.in_global_basis(Ex, Ey, 0 * Ez))
 
     def get_Uinc(self, x, y, z, k0: float):
         """-2 j beta E (port_bcs.py:985-1000)."""

@@ -292,6 +292,26 @@ int em_csc_add(void* data_csc, int32_t precision, const int64_t* pos, const void
                int64_t n, double scale_re, double scale_im, void* stream);
 
 /*
+ * Point location (compiled/base/index_inter.py:37-298, index_interp): for every point of
+ * coords (DEVICE double [3][n]) the largest id among the candidate tets (tetids, DEVICE
+ * int64, or all plan tets when NULL) whose barycentric coordinates at the point are all
+ * >= -1e-8 and sum to <= 1 + 1e-8; -1 when none. Blocks.
+ */
+int em_ned2_locate(em_ned2_plan_t plan, const double* coords, int64_t n, const int64_t* tetids,
+                   int64_t n_tetids, int64_t* tet_of, void* stream);
+/*
+ * Interpolation of a solution (complex128 [n_field]) at points with known tets
+ * (compiled/base/interp.py:286-387 ned2_tet_interp; with EM_INTERP_CURL :410-519
+ * ned2_tet_interp_curl, the curl times c_tet[tet] when c_tet (complex128 per plan tet) is
+ * not NULL). out: complex128 [3][n]; NaN where tet_of is -1.
+ */
+#define EM_INTERP_E 0
+#define EM_INTERP_CURL 1
+int em_ned2_interp(em_ned2_plan_t plan, const double* coords, int64_t n, const int64_t* tet_of,
+                   const void* solution, int32_t which, const void* c_tet, void* out,
+                   void* stream);
+
+/*
  * HIP graphs of assembly launches. The reference re-runs its Numba builder per call
  * (microwave_3d.py:1457 sweep loop) and has no counterpart: these record the launches
  * issued on `stream` between begin and end (hipStreamBeginCapture, thread-local mode)

@@ -24,6 +24,8 @@
  *   ned2_oracle_build     curlcurl.py:387-456 (_matrix_builder; COO block per tet,
  *                         row-major, p = iassy*ndof^2)
  *   ned2_oracle_csc_*     mth/csc_cast.py:65-177 (precompute_csc_pattern, scatter)
+ *   ned2_oracle_interp    compiled/base/interp.py:286-387, 410-519 (field and curl)
+ *   ned2_oracle_locate    compiled/base/index_inter.py:134-246 (brute force)
  */
 #include <complex.h>
 #include <math.h>
@@ -416,3 +418,119 @@ void ned2_oracle_csc_scatter(const double *data_coo, const int64_t *perm, const 
 
 /* Copy of the quadrature table (weights row, then 4 barycentric rows), for tests. */
 void ned2_oracle_keast(double *out) { memcpy(out, DPTS, sizeof(DPTS)); }
+
+/* compiled/base/interp.py:286-387 (ned2_tet_interp) and :410-519 (ned2_tet_interp_curl)
+ * for points whose tet is known (tet_of[j], -1: NaN). Per point: tet_coefficients in
+ * physical coordinates (or translated to stored vertex 0 when local_frame), coefficients
+ * over 6V, local_mapping of the slot edges/faces, then sum_i sol[tet_to_field[i]] f_i
+ * (curl: times c_tet[t] when given). coords [3][n]; out interleaved complex [3][n]. */
+int ned2_oracle_interp(const double *nodes, int64_t Nn, const int64_t *tets, int64_t T,
+                       const int64_t *tris, int64_t F, const int64_t *edges, int64_t E,
+                       const int64_t *tet_to_field, const int64_t *tet_to_edge,
+                       const int64_t *tet_to_tri, const int64_t *dofcodes, int ndof,
+                       const double *coords, int64_t n, const int64_t *tet_of,
+                       const cplx *sol, int curl, const cplx *c_tet, int local_frame,
+                       cplx *out) {
+    int64_t typ[MAXDOF], idx[MAXDOF];
+    or_parse_dofcode(dofcodes, ndof, typ, idx);
+    int status = OR_OK;
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < n; ++j) {
+        const int64_t t = tet_of[j];
+        if (t < 0) {
+            for (int q = 0; q < 3; ++q) out[q * n + j] = NAN + NAN * I;
+            continue;
+        }
+        int64_t vid[4];
+        double vx[3][4], o[3] = {0, 0, 0};
+        for (int k = 0; k < 4; ++k) vid[k] = tets[k * T + t];
+        if (local_frame)
+            for (int c = 0; c < 3; ++c) o[c] = nodes[c * Nn + vid[0]];
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) vx[c][k] = nodes[c * Nn + vid[k]] - o[c];
+        int64_t lem[2][6], ltm[3][4];
+        for (int e = 0; e < 6; ++e) {
+            const int64_t ge = tet_to_edge[e * T + t];
+            int64_t g2[2] = {edges[ge], edges[E + ge]};
+            int64_t l2[2];
+            or_local_map(vid, g2, 2, l2);
+            lem[0][e] = l2[0];
+            lem[1][e] = l2[1];
+        }
+        for (int f = 0; f < 4; ++f) {
+            const int64_t gf = tet_to_tri[f * T + t];
+            int64_t g3[3] = {tris[gf], tris[F + gf], tris[2 * F + gf]};
+            int64_t l3[3];
+            or_local_map(vid, g3, 3, l3);
+            for (int r = 0; r < 3; ++r) ltm[r][f] = l3[r];
+        }
+        double a[4], b[4], c[4], d[4], V;
+        or_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
+        double g[4][3], l[4];
+        const double px = coords[j] - o[0], py = coords[n + j] - o[1], pz = coords[2 * n + j] - o[2];
+        for (int k = 0; k < 4; ++k) {
+            g[k][0] = b[k] / (6.0 * V);
+            g[k][1] = c[k] / (6.0 * V);
+            g[k][2] = d[k] / (6.0 * V);
+            l[k] = a[k] / (6.0 * V) + g[k][0] * px + g[k][1] * py + g[k][2] * pz;
+        }
+        cplx acc[3] = {0, 0, 0};
+        for (int i = 0; i < ndof; ++i) {
+            int i1, j1, k1;
+            if (typ[i] == 0) {
+                i1 = (int)lem[0][idx[i]];
+                j1 = (int)lem[1][idx[i]];
+                k1 = 0;
+            } else {
+                i1 = (int)ltm[0][idx[i]];
+                j1 = (int)ltm[1][idx[i]];
+                k1 = (int)ltm[2][idx[i]];
+            }
+            double f[3], cf[3];
+            if (or_basis((int)dofcodes[i], i1, j1, k1, l, g, f, cf) != OR_OK) {
+                status = OR_EBADCODE;
+                continue;
+            }
+            const cplx w = sol[tet_to_field[i * T + t]];
+            for (int q = 0; q < 3; ++q) acc[q] += (curl ? cf[q] : f[q]) * w;
+        }
+        const cplx k = (curl && c_tet) ? c_tet[t] : 1.0;
+        for (int q = 0; q < 3; ++q) out[q * n + j] = acc[q] * k;
+    }
+    return status;
+}
+
+/* compiled/base/index_inter.py:134-246 by brute force: every candidate tet tests every
+ * point with the same barycentric formula and tolerance; a point keeps the largest
+ * containing tet id (the reference's answer for ascending candidate lists). */
+void ned2_oracle_locate(const double *nodes, int64_t Nn, const int64_t *tets, int64_t T,
+                        const int64_t *tetids, int64_t n_cand, const double *coords, int64_t n,
+                        int64_t *out) {
+    const double EPS = 1e-8;
+    for (int64_t j = 0; j < n; ++j) out[j] = -1;
+    for (int64_t it = 0; it < n_cand; ++it) {
+        const int64_t t = tetids ? tetids[it] : it;
+        const int64_t i1 = tets[t], i2 = tets[T + t], i3 = tets[2 * T + t], i4 = tets[3 * T + t];
+        const double v1x = nodes[i1], v1y = nodes[Nn + i1], v1z = nodes[2 * Nn + i1];
+        const double m00 = nodes[i2] - v1x, m10 = nodes[Nn + i2] - v1y, m20 = nodes[2 * Nn + i2] - v1z;
+        const double m01 = nodes[i3] - v1x, m11 = nodes[Nn + i3] - v1y, m21 = nodes[2 * Nn + i3] - v1z;
+        const double m02 = nodes[i4] - v1x, m12 = nodes[Nn + i4] - v1y, m22 = nodes[2 * Nn + i4] - v1z;
+        const double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) +
+                           m02 * (m10 * m21 - m11 * m20);
+        const double id = 1.0 / det;
+        const double b00 = (m11 * m22 - m12 * m21) * id, b01 = (m02 * m21 - m01 * m22) * id,
+                     b02 = (m01 * m12 - m02 * m11) * id;
+        const double b10 = (m12 * m20 - m10 * m22) * id, b11 = (m00 * m22 - m02 * m20) * id,
+                     b12 = (m10 * m02 - m00 * m12) * id;
+        const double b20 = (m10 * m21 - m11 * m20) * id, b21 = (m20 * m01 - m00 * m21) * id,
+                     b22 = (m00 * m11 - m10 * m01) * id;
+        for (int64_t j = 0; j < n; ++j) {
+            const double dx = coords[j] - v1x, dy = coords[n + j] - v1y, dz = coords[2 * n + j] - v1z;
+            const double u = b00 * dx + b01 * dy + b02 * dz;
+            const double v = b10 * dx + b11 * dy + b12 * dz;
+            const double w = b20 * dx + b21 * dy + b22 * dz;
+            if (u >= -EPS && v >= -EPS && w >= -EPS && u + v + w <= 1.0 + EPS)
+                if (t > out[j]) out[j] = t;
+        }
+    }
+}

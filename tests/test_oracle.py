@@ -198,7 +198,9 @@ def test_local_frame_oracle_matches_exact_on_slivers():
         Kx = np.array([[float(x) for x in r] for r in K])
         worst_loc = max(worst_loc, rel_fro(dE_loc[400 * t:400 * t + 400].reshape(20, 20), Kx))
         worst_abs = max(worst_abs, rel_fro(dE_abs[400 * t:400 * t + 400].reshape(20, 20), Kx))
-    assert worst_loc < 1e-13
+    # the oracle is built with -ffast-math (the reference element kernel's fastmath=True),
+    # so the last bits move with code generation: 1.1e-13 seen, bound 5e-13
+    assert worst_loc < 5e-13
     assert worst_abs < 1e-8
 
 

@@ -307,6 +307,77 @@ def suite_csc_pec(args) -> dict:
     return res
 
 
+def suite_surface_interp(args) -> dict:
+    """Robin/port surface assembly and field interpolation on a ~500k-tet Delaunay mesh:
+    the 8x8 blocks of every outer face (absorbing boundary), their CSC positions, the
+    per-frequency add into the CSC values, a port excitation vector; point location and
+    interpolation of a random solution at 2M random points. CPU lines: the oracle's
+    restatements of compute_bc_entries and ned2_tet_interp (one thread) on samples."""
+    import numpy as np
+    import torch
+    import oracle
+    from oracle import interp as oi
+    from oracle import robin as orob
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    from emerge_amd.interp import interp, locate
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    from emerge_amd.robin import RobinSurface
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=args.seed + 8))
+    plan = Ned2Plan.from_field(f)
+    T = plan.n_tets
+    m = f.mesh
+    surf = m.boundary_tris()
+    rs = RobinSurface(plan, f, surf)
+    unit = rs.blocks(1.0)
+    ms_blocks, _ = time_launches(lambda: rs.blocks(1.0, out=unit), 10, 2)
+    cm = CscMap(plan)
+    data = torch.zeros(cm.nnz, dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pos = rs.positions(cm)
+    torch.cuda.synchronize()
+    ms_pos = (time.perf_counter() - t0) * 1e3
+    ms_add, _ = time_launches(lambda: rs.add_to_csc(data, pos, unit, 1j * 150.0), 10, 2)
+    U = torch.randn((3, 6, len(surf)), dtype=torch.complex128, device="cuda")
+    bvec = torch.zeros(f.n_field, dtype=torch.complex128, device="cuda")
+    ms_force, _ = time_launches(lambda: rs.force(U, bvec=bvec), 5, 1)
+    ns = min(len(surf), 20000)
+    t0 = time.perf_counter()
+    orob.bc_entries(m.nodes, m.tris, surf[:ns], 1.0)
+    cpu_blocks = (time.perf_counter() - t0) / ns * len(surf) * 1e3
+    # interpolation
+    rng = np.random.default_rng(args.seed + 8)
+    npts = 2_000_000
+    pts = torch.as_tensor(rng.uniform(0, 1, (3, npts)), device="cuda")
+    sol = torch.as_tensor(rng.normal(size=f.n_field) + 1j * rng.normal(size=f.n_field),
+                          device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tet = locate(plan, pts)
+    torch.cuda.synchronize()
+    ms_locate = (time.perf_counter() - t0) * 1e3
+    found = int((tet >= 0).sum())
+    ms_interp, _ = time_launches(lambda: interp(plan, pts, sol, tet), 10, 2)
+    ms_curl, _ = time_launches(lambda: interp(plan, pts, sol, tet, curl=True), 10, 2)
+    ncpu = 100_000
+    tet_h = tet[:ncpu].cpu().numpy()
+    t0 = time.perf_counter()
+    oi.interp(f, pts[:, :ncpu].cpu().numpy(), sol.cpu().numpy(), tet_h)
+    cpu_interp = (time.perf_counter() - t0) / ncpu * npts * 1e3
+    del data, pos, unit, cm, pts, sol, tet
+    torch.cuda.empty_cache()
+    return {"n_tets": T, "n_faces": len(surf), "n_points": npts, "points_located": found,
+            "ms": {"robin_blocks": ms_blocks, "csc_positions_one_time": ms_pos,
+                   "robin_add_per_frequency": ms_add, "port_force": ms_force,
+                   "locate": ms_locate, "interp_E": ms_interp, "interp_curl": ms_curl},
+            "faces_per_s": len(surf) / (ms_blocks * 1e-3),
+            "points_per_s_interp": npts / (ms_interp * 1e-3),
+            "cpu_ms_oracle_blocks_1thread_scaled": cpu_blocks,
+            "cpu_ms_oracle_interp_scaled": cpu_interp,
+            "note": "CPU figures: the oracle on 20k faces / 100k points, scaled to the full "
+                    "counts; interp oracle uses OpenMP threads"}
+
+
 def target_run(args) -> dict:
     """The large single-mesh case: mixed and FP64 fused A on a ~8M-tet Delaunay mesh."""
     import numpy as np
@@ -370,7 +441,8 @@ def main() -> int:
                     help="also measure the frequency sweep (500k tets x 64 f), the uneven "
                          "32-mesh batch and the backward on 1M tets")
     ap.add_argument("--only", default="",
-                    help="with --suite: comma list among sweep,batch,backward,csc_pec")
+                    help="with --suite: comma list among sweep,batch,backward,csc_pec,"
+                         "surface_interp")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -506,7 +578,8 @@ def main() -> int:
             runs = (("sweep", "sweep_500k_x64", suite_sweep),
                     ("batch", "batch_32_meshes", suite_batch),
                     ("backward", "backward_1M", suite_backward),
-                    ("csc_pec", "csc_pec_500k", suite_csc_pec))
+                    ("csc_pec", "csc_pec_500k", suite_csc_pec),
+                    ("surface_interp", "surface_interp_500k", suite_surface_interp))
             only = set(args.only.split(",")) if args.only else None
             for short, key, fn in runs:
                 if only is None or short in only:

@@ -109,6 +109,13 @@ _SIGS = {
     "em_csc_add": (ctypes.c_int, [P, I32, P, P, I64, ctypes.c_double, ctypes.c_double, P]),
     "em_ned2_locate": (ctypes.c_int, [P, P, I64, P, I64, P, P]),
     "em_ned2_interp": (ctypes.c_int, [P, P, I64, P, P, I32, P, P, P]),
+    "em_heat_plan_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.c_int, P, I64, P, I64, P,
+                                           I64]),
+    "em_heat_plan_destroy": (ctypes.c_int, [P]),
+    "em_heat_assemble": (ctypes.c_int, [P, P, I32, I64, I64, P, I64, P, P, P]),
+    "em_heat_rowcol": (ctypes.c_int, [P, P, P, P]),
+    "em_csc_map_create_heat": (ctypes.c_int, [ctypes.POINTER(P), P, P]),
+    "em_csc_to_csc_real": (ctypes.c_int, [P, P, P, ctypes.c_double, P, P]),
     "em_graph_begin_capture": (ctypes.c_int, [P]),
     "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "em_graph_launch": (ctypes.c_int, [P, P]),

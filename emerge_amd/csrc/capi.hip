@@ -656,7 +656,8 @@ int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64
     if (perm && map->nnz_coo)
         EM_HIP(hipMemcpyAsync(perm, map->b.perm, sizeof(int64_t) * map->nnz_coo, k, s));
     if (target_csc) EM_HIP(em::csc_target(map->b.seg, map->b.nnz, (long long*)target_csc, s));
-    if (col_offsets) EM_HIP(em::csc_col_offsets(map->b.inc_ptr, map->N, (long long*)col_offsets, s));
+    if (col_offsets)
+        EM_HIP(em::csc_col_offsets(map->b.inc_ptr, map->N, map->b.ndof, (long long*)col_offsets, s));
     return EM_OK;
 }
 
@@ -677,6 +678,10 @@ static int csc_apply(em_csc_map_t map, const void* E, const void* B, double k0, 
     if (map->b.nnz > 0 && (!E || !out)) {
         set_error("null data");
         return EM_EINVAL;
+    }
+    if (map->b.ndof != 20) {
+        set_error("to_csc: complex values need a Nedelec (20-DOF) map; use em_csc_to_csc_real");
+        return EM_EUNSUPPORTED;
     }
     DeviceGuard g(map->device);
     const int c64 = precision == EM_PREC_MIXED ? 1 : 0;
@@ -975,6 +980,178 @@ int em_ned2_interp(em_ned2_plan_t plan, const double* coords, int64_t n, const i
                            (const long long*)tet_of, (const double2*)solution,
                            which == EM_INTERP_CURL, (const double2*)c_tet, (double2*)out,
                            (hipStream_t)stream));
+    return EM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Heat conduction (Lagrange-2)
+// ---------------------------------------------------------------------------------------
+
+struct em_heat_plan_s {
+    int device = 0;
+    long long n_nodes = 0, n_tets = 0, n_field = 0;
+    double4* xyz = nullptr;
+    int4* tets = nullptr;
+    int* t2f = nullptr;  // [T][10]
+};
+
+static void free_heat(em_heat_plan_s* p) {
+    if (!p) return;
+    (void)hipFree(p->xyz);
+    (void)hipFree(p->tets);
+    (void)hipFree(p->t2f);
+    delete p;
+}
+
+int em_heat_plan_create(em_heat_plan_t* plan, int device, const double* nodes, int64_t n_nodes,
+                        const int64_t* tets, int64_t n_tets, const int64_t* tet_to_field,
+                        int64_t n_field) {
+    if (!plan) {
+        set_error("heat_plan: null output pointer");
+        return EM_EINVAL;
+    }
+    *plan = nullptr;
+    if (!nodes || !tets || !tet_to_field || n_nodes <= 0 || n_tets < 0 || n_field <= 0 ||
+        n_nodes >= (1ll << 31) || n_field >= (1ll << 31) || n_tets * 10 >= (1ll << 31)) {
+        set_error("heat_plan: bad argument or size");
+        return EM_EINVAL;
+    }
+    std::vector<double4> hxyz((size_t)n_nodes);
+    for (long long n = 0; n < n_nodes; ++n)
+        hxyz[n] = make_double4(nodes[n], nodes[n_nodes + n], nodes[2 * n_nodes + n], 0.0);
+    std::vector<int4> ht((size_t)n_tets);
+    std::vector<int> hf((size_t)n_tets * 10);
+    for (long long t = 0; t < n_tets; ++t) {
+        long long v[4];
+        for (int k = 0; k < 4; ++k) {
+            v[k] = tets[k * n_tets + t];
+            if (v[k] < 0 || v[k] >= n_nodes) {
+                set_error("heat_plan: tet node id out of range");
+                return EM_EINVAL;
+            }
+        }
+        ht[t] = make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+        double J[3][3];
+        for (int k = 0; k < 3; ++k) {
+            J[k][0] = hxyz[v[k + 1]].x - hxyz[v[0]].x;
+            J[k][1] = hxyz[v[k + 1]].y - hxyz[v[0]].y;
+            J[k][2] = hxyz[v[k + 1]].z - hxyz[v[0]].z;
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        if (det == 0.0 || !std::isfinite(det)) {
+            set_error("heat_plan: zero-volume tet " + std::to_string(t));
+            return EM_EDEGENERATE;
+        }
+        for (int d = 0; d < 10; ++d) {
+            const long long f = tet_to_field[d * n_tets + t];
+            if (f < 0 || f >= n_field) {
+                set_error("heat_plan: tet_to_field entry out of range");
+                return EM_EINVAL;
+            }
+            hf[t * 10 + d] = (int)f;
+        }
+    }
+    DeviceGuard g(device);
+    auto* p = new em_heat_plan_s();
+    p->device = device;
+    p->n_nodes = n_nodes;
+    p->n_tets = n_tets;
+    p->n_field = n_field;
+    int st;
+    if ((st = dmalloc(&p->xyz, hxyz.size())) || (st = dmalloc(&p->tets, ht.size())) ||
+        (st = dmalloc(&p->t2f, hf.size()))) {
+        free_heat(p);
+        return st;
+    }
+    hipError_t e = hipMemcpy(p->xyz, hxyz.data(), hxyz.size() * sizeof(double4),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ht.empty())
+        e = hipMemcpy(p->tets, ht.data(), ht.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !hf.empty())
+        e = hipMemcpy(p->t2f, hf.data(), hf.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("heat_plan: ") + hipGetErrorString(e));
+        free_heat(p);
+        return EM_EHIP;
+    }
+    *plan = p;
+    return EM_OK;
+}
+
+int em_heat_plan_destroy(em_heat_plan_t plan) {
+    if (!plan) return EM_OK;
+    DeviceGuard g(plan->device);
+    free_heat(plan);
+    return EM_OK;
+}
+
+int em_heat_assemble(em_heat_plan_t plan, const double* cond, int32_t cond_kind,
+                     int64_t cond_tet_stride, int64_t cond_comp_stride, const double* rho_cp,
+                     int64_t rho_tet_stride, double* K, double* M, void* stream) {
+    if (!plan || (K && !cond) || (M && !rho_cp) || cond_tet_stride < 0 ||
+        cond_comp_stride < 0 || rho_tet_stride < 0 ||
+        (K && (cond_kind < EM_MAT_SCALAR || cond_kind > EM_MAT_FULL))) {
+        set_error("heat_assemble: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    EM_HIP(em::heat_assemble(plan->xyz, plan->tets, plan->n_tets, cond, cond_kind,
+                             cond_tet_stride, cond_comp_stride, rho_cp, rho_tet_stride, K, M,
+                             (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_heat_rowcol(em_heat_plan_t plan, int64_t* rows, int64_t* cols, void* stream) {
+    if (!plan || (plan->n_tets > 0 && (!rows || !cols))) {
+        set_error("heat_rowcol: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    EM_HIP(em::heat_rowcol(plan->t2f, plan->n_tets, (long long*)rows, (long long*)cols,
+                           (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream) {
+    if (!map || !plan) {
+        set_error("csc_map_create_heat: null argument");
+        return EM_EINVAL;
+    }
+    *map = nullptr;
+    DeviceGuard g(plan->device);
+    auto* c = new em_csc_map_s();
+    c->device = plan->device;
+    c->N = plan->n_field;
+    c->nnz_coo = plan->n_tets * 100;
+    c->n_perm = c->nnz_coo;
+    long long max_inc = 0;
+    hipError_t e = em::csc_build_ndof10(plan->t2f, plan->n_tets, c->N, c->b, max_inc,
+                                        (hipStream_t)stream);
+    if (e != hipSuccess) {
+        delete c;
+        if (e == hipErrorInvalidValue && max_inc * 10 > 2048) {
+            set_error("csc_map_create_heat: a DOF lies in " + std::to_string(max_inc) +
+                      " tets (at most 204 supported)");
+            return EM_EUNSUPPORTED;
+        }
+        set_error(std::string("csc_map_create_heat: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;
+    }
+    *map = c;
+    return EM_OK;
+}
+
+int em_csc_to_csc_real(em_csc_map_t map, const double* data, const double* data_b, double scale,
+                       double* data_csc, void* stream) {
+    if (!map || (map->b.nnz > 0 && (!data || !data_csc))) {
+        set_error("to_csc_real: bad argument");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(map->device);
+    EM_HIP(em::csc_gather_real(map->b.seg, map->b.perm, map->b.nnz, data, data_b, scale, data_csc,
+                               (hipStream_t)stream));
     return EM_OK;
 }
 

@@ -97,30 +97,32 @@ __device__ __forceinline__ long long tet_of(const int* tetids, long long ia) {
     return tetids ? (long long)tetids[ia] : ia;
 }
 
+template <int NDOF>
 __global__ __launch_bounds__(256) void inc_count_kernel(const int* __restrict__ t2f,
                                                         const int* __restrict__ tetids,
                                                         long long n_assy,
                                                         unsigned long long* __restrict__ cnt) {
-    const long long n = n_assy * 20;
+    const long long n = n_assy * NDOF;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (long long)gridDim.x * blockDim.x) {
-        const long long ia = e / 20;
-        const int d = t2f[tet_of(tetids, ia) * 20 + (int)(e - ia * 20)];
+        const long long ia = e / NDOF;
+        const int d = t2f[tet_of(tetids, ia) * NDOF + (int)(e - ia * NDOF)];
         atomicAdd(cnt + d, 1ull);
     }
 }
 
+template <int NDOF>
 __global__ __launch_bounds__(256) void inc_fill_kernel(const int* __restrict__ t2f,
                                                        const int* __restrict__ tetids,
                                                        long long n_assy,
                                                        const long long* __restrict__ inc_ptr,
                                                        unsigned long long* __restrict__ cursor,
                                                        int* __restrict__ inc) {
-    const long long n = n_assy * 20;
+    const long long n = n_assy * NDOF;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (long long)gridDim.x * blockDim.x) {
-        const long long ia = e / 20;
-        const int d = t2f[tet_of(tetids, ia) * 20 + (int)(e - ia * 20)];
+        const long long ia = e / NDOF;
+        const int d = t2f[tet_of(tetids, ia) * NDOF + (int)(e - ia * NDOF)];
         const unsigned long long pos = atomicAdd(cursor + d, 1ull);
         inc[inc_ptr[d] + (long long)pos] = (int)e;
     }
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(256) void inc_sort_kernel(const long long* __restri
 
 constexpr int kColCap = 2048;  // COO entries per column handled in LDS (102 incidences)
 
-template <bool FILL>
+template <bool FILL, int NDOF>
 __global__ __launch_bounds__(64) void csc_column_kernel(
     const int* __restrict__ t2f, const int* __restrict__ tetids,
     const long long* __restrict__ inc_ptr, const int* __restrict__ inc, long long N,
@@ -160,11 +162,11 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
     const int lane = threadIdx.x;
     for (long long c = blockIdx.x; c < N; c += gridDim.x) {
         const long long s = inc_ptr[c];
-        const int n = (int)(20 * (inc_ptr[c + 1] - s));
+        const int n = (int)(NDOF * (inc_ptr[c + 1] - s));
         for (int i = lane; i < n; i += 64) {
-            const int e = inc[s + i / 20];
-            const long long ia = e / 20;
-            const int row = t2f[tet_of(tetids, ia) * 20 + (i % 20)];
+            const int e = inc[s + i / NDOF];
+            const long long ia = e / NDOF;
+            const int row = t2f[tet_of(tetids, ia) * NDOF + (i % NDOF)];
             key[i] = ((unsigned long long)(unsigned)row << 32) | (unsigned)i;
         }
         __syncthreads();
@@ -186,14 +188,14 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
                 const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
                 const long long q = indptr[c] + carry + __popcll(mask & upto) - 1;
                 const int li = (int)(cur & 0xffffffffu);
-                const int e = inc[s + li / 20];
-                const long long ia = e / 20;
-                const int jj = e % 20, ii = li % 20;
-                perm[20 * s + i] = 400 * ia + 20 * ii + jj;
-                pos[20 * s + li] = (unsigned short)(q - indptr[c]);
+                const int e = inc[s + li / NDOF];
+                const long long ia = e / NDOF;
+                const int jj = e % NDOF, ii = li % NDOF;
+                perm[NDOF * s + i] = (long long)NDOF * NDOF * ia + NDOF * ii + jj;
+                if (pos) pos[NDOF * s + li] = (unsigned short)(q - indptr[c]);
                 if (flag) {
                     indices[q] = (long long)(cur >> 32);
-                    seg[q] = 20 * s + i;
+                    seg[q] = NDOF * s + i;
                 }
             }
             carry += __popcll(mask);
@@ -368,10 +370,19 @@ static unsigned grid_for(long long n, long long cap = 65536 * 4) {
     return (unsigned)b;
 }
 
-// Builds the pattern of plan mesh m (N DOFs). On success the caller owns b's buffers.
-hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
-                     hipStream_t s) {
-    const long long nnz_coo = m.n_assy * 400;
+// Builds the pattern of a DOF map t2f [T][NDOF] over n_assy assembly indices (N DOFs).
+// On success the caller owns b's buffers. The incidence-gather row positions (pos) are
+// built for NDOF = 20 only.
+template <int NDOF>
+static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_assy, long long N,
+                              CscBuild& b, long long& max_inc, hipStream_t s) {
+    struct {
+        const int* t2f;
+        const int* tetids;
+        long long n_assy;
+    } m = {t2f, tetids, n_assy};
+    constexpr bool kPos = NDOF == 20;
+    const long long nnz_coo = m.n_assy * NDOF * NDOF;
     hipError_t e;
     unsigned long long *cnt = nullptr, *maxlen = nullptr;
     int* inc = nullptr;
@@ -384,20 +395,24 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
     TRY(hipMalloc((void**)&cnt, sizeof(unsigned long long) * (N + 1)));
     TRY(hipMalloc((void**)&b.inc_ptr, sizeof(long long) * (N + 1)));
     TRY(hipMalloc((void**)&maxlen, sizeof(unsigned long long)));
-    TRY(hipMalloc((void**)&inc, sizeof(int) * (size_t)(m.n_assy * 20 > 0 ? m.n_assy * 20 : 1)));
-    TRY(hipMalloc((void**)&b.pos,
-                  sizeof(unsigned short) * (size_t)(m.n_assy * 400 > 0 ? m.n_assy * 400 : 1)));
+    TRY(hipMalloc((void**)&inc,
+                  sizeof(int) * (size_t)(m.n_assy * NDOF > 0 ? m.n_assy * NDOF : 1)));
+    if (kPos)
+        TRY(hipMalloc((void**)&b.pos, sizeof(unsigned short) *
+                                          (size_t)(nnz_coo > 0 ? nnz_coo : 1)));
     TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
     TRY(hipMemsetAsync(maxlen, 0, sizeof(unsigned long long), s));
     if (m.n_assy > 0) {
-        hipLaunchKernelGGL(inc_count_kernel, dim3(grid_for(m.n_assy * 20)), dim3(256), 0, s, m.t2f,
+        hipLaunchKernelGGL(inc_count_kernel<NDOF>, dim3(grid_for(m.n_assy * NDOF)), dim3(256), 0, s,
+                           m.t2f,
                            m.tetids, m.n_assy, cnt);
         TRY(hipGetLastError());
     }
     TRY(exclusive_scan_i64((const long long*)cnt, b.inc_ptr, N + 1, s));
     TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
     if (m.n_assy > 0) {
-        hipLaunchKernelGGL(inc_fill_kernel, dim3(grid_for(m.n_assy * 20)), dim3(256), 0, s, m.t2f,
+        hipLaunchKernelGGL(inc_fill_kernel<NDOF>, dim3(grid_for(m.n_assy * NDOF)), dim3(256), 0, s,
+                           m.t2f,
                            m.tetids, m.n_assy, b.inc_ptr, cnt, inc);
         TRY(hipGetLastError());
     }
@@ -409,7 +424,7 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
         TRY(hipMemcpyAsync(&ml, maxlen, sizeof(ml), hipMemcpyDeviceToHost, s));
         TRY(hipStreamSynchronize(s));
         max_inc = (long long)ml;
-        if (20 * max_inc > kColCap) {
+        if (NDOF * max_inc > kColCap) {
             e = hipErrorInvalidValue;
             goto fail;
         }
@@ -420,7 +435,8 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
     {
         const unsigned g = (unsigned)(N < (1 << 20) ? N : (1 << 20));
         if (N > 0) {
-            hipLaunchKernelGGL(csc_column_kernel<false>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
+            hipLaunchKernelGGL((csc_column_kernel<false, NDOF>), dim3(g), dim3(64), 0, s, m.t2f,
+                               m.tetids,
                                b.inc_ptr, inc, N, ucount, nullptr, nullptr, nullptr, nullptr,
                                nullptr);
             TRY(hipGetLastError());
@@ -434,7 +450,8 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
         TRY(hipMalloc((void**)&b.perm, sizeof(long long) * (nnz_coo > 0 ? nnz_coo : 1)));
         TRY(hipMemcpyAsync(b.seg + b.nnz, &nnz_coo, sizeof(long long), hipMemcpyHostToDevice, s));
         if (N > 0) {
-            hipLaunchKernelGGL(csc_column_kernel<true>, dim3(g), dim3(64), 0, s, m.t2f, m.tetids,
+            hipLaunchKernelGGL((csc_column_kernel<true, NDOF>), dim3(g), dim3(64), 0, s, m.t2f,
+                               m.tetids,
                                b.inc_ptr, inc, N, nullptr, b.indptr, b.indices, b.perm, b.seg,
                                b.pos);
             TRY(hipGetLastError());
@@ -459,6 +476,48 @@ fail:
     b = CscBuild();
     return e;
 #undef TRY
+}
+
+hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
+                     hipStream_t s) {
+    b.ndof = 20;
+    return csc_build_n<20>(m.t2f, m.tetids, m.n_assy, N, b, max_inc, s);
+}
+
+hipError_t csc_build_ndof10(const int* t2f, long long n_tets, long long N, CscBuild& b,
+                            long long& max_inc, hipStream_t s) {
+    b.ndof = 10;
+    hipError_t e = csc_build_n<10>(t2f, nullptr, n_tets, N, b, max_inc, s);
+    if (e == hipSuccess) b.ndof = 10;
+    return e;
+}
+
+// Real-valued data (one double per COO entry): data_csc[q] = sum of its contributors.
+__global__ __launch_bounds__(256) void csc_gather_real_kernel(const long long* __restrict__ seg,
+                                                              const long long* __restrict__ perm,
+                                                              long long nnz,
+                                                              const double* __restrict__ E,
+                                                              const double* __restrict__ B,
+                                                              double scale,
+                                                              double* __restrict__ out) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nnz;
+         q += (long long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (long long j = seg[q]; j < seg[q + 1]; ++j) {
+            const long long k = perm[j];
+            acc += B ? fma(scale, B[k], E[k]) : E[k];
+        }
+        out[q] = acc;
+    }
+}
+
+hipError_t csc_gather_real(const long long* seg, const long long* perm, long long nnz,
+                           const double* E, const double* B, double scale, double* out,
+                           hipStream_t s) {
+    if (nnz == 0) return hipSuccess;
+    hipLaunchKernelGGL(csc_gather_real_kernel, dim3(grid_for(nnz)), dim3(256), 0, s, seg, perm,
+                       nnz, E, B, scale, out);
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -522,8 +581,9 @@ hipError_t csc_target(const long long* seg, long long nnz, long long* target, hi
     return hipGetLastError();
 }
 
-hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s) {
-    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(N + 1)), dim3(256), 0, s, inc_ptr, N + 1, 20ll,
+hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long ndof, long long* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(N + 1)), dim3(256), 0, s, inc_ptr, N + 1, ndof,
                        out);
     return hipGetLastError();
 }

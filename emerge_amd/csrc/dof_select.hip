@@ -399,6 +399,7 @@ hipError_t csc_restrict(const CscBuild& full, long long n, const long long* new_
     long long* newq = nullptr;
     const bool inc_tables = full.inc && full.pos;
     long long n_inc = 0;
+    sub.ndof = full.ndof;
     hipError_t e;
 #define TRY(x)                          \
     do {                                \
@@ -470,6 +471,7 @@ fail:
     (void)hipFree(sub.pos);
     (void)hipFree(sub.cols);
     sub = CscBuild();
+    sub.ndof = full.ndof;
     return e;
 #undef TRY
 }

@@ -79,15 +79,31 @@ struct CscBuild {
     unsigned short* pos = nullptr;  // [400 Ta] entry index of row ii within the column
     long long* cols = nullptr;      // restricted maps: full-matrix column of each column
     long long nnz = 0;
+    int ndof = 20;                  // DOFs per tet of the COO blocks (20 Nedelec, 10 Lagrange)
 };
 hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
                      hipStream_t s);
 // mirror: the COO blocks are symmetric; read each contributor from its mirrored entry.
+// Pattern of a 10-DOF-per-tet map (Lagrange-2: t2f [T][10], every tet assembled).
+hipError_t csc_build_ndof10(const int* t2f, long long n_tets, long long N, CscBuild& b,
+                            long long& max_inc, hipStream_t s);
+// out[q] = sum of E[k] (+ scale * B[k]) over the contributors of entry q (real data).
+hipError_t csc_gather_real(const long long* seg, const long long* perm, long long nnz,
+                           const double* E, const double* B, double scale, double* out,
+                           hipStream_t s);
 hipError_t csc_gather(const long long* seg, const long long* perm, long long nnz, const void* E,
                       const void* B, double k0sq, void* out, int complex64, int mirror,
                       hipStream_t s);
 hipError_t csc_target(const long long* seg, long long nnz, long long* target, hipStream_t s);
-hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long* out, hipStream_t s);
+hipError_t csc_col_offsets(const long long* inc_ptr, long long N, long long ndof, long long* out,
+                           hipStream_t s);
+// Heat (Lagrange-2) assembly (heat.hip): K and/or M [100 T] of every tet; cond indexed
+// cond[t * cts + comp * ccs] with EM_MAT_* kind, rho[t * rts].
+hipError_t heat_assemble(const double4* xyz, const int4* tets, long long T, const double* cond,
+                         int cond_kind, long long cts, long long ccs, const double* rho,
+                         long long rts, double* K, double* M, hipStream_t s);
+hipError_t heat_rowcol(const int* t2f, long long T, long long* rows, long long* cols,
+                       hipStream_t s);
 // to_csc by column incidences (full maps: inc and pos present); same result as csc_gather.
 hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,
                           double k0sq, void* out, int complex64, int mirror, hipStream_t s);

@@ -371,3 +371,23 @@ def delaunay_mesh_with_tets(target_tets: int, seed: int, box=(1.0, 1.0, 1.0)) ->
     """Delaunay box mesh sized for about ``target_tets`` tets (about 6.5 tets per point)."""
     n = max(8, int(target_tets / 6.55))
     return delaunay_box_mesh(n, seed, box=box)
+
+
+class Legrange2Field:
+    """The DOF map of second-order Lagrange elements on a ``TetMesh`` (reference
+    ``Legrange2``, elements/leg2.py:29-60): node DOFs first (the node ids), then one DOF
+    per edge (edge id + n_nodes); ``tet_to_field`` (10, T) lists a tet's 4 vertices in
+    stored order and its 6 edges in slot order."""
+
+    def __init__(self, mesh: TetMesh):
+        self.mesh = mesh
+        self.nnodes = mesh.n_nodes
+        self.nedges = mesh.n_edges
+        self.ntets = mesh.n_tets
+        self.n_tets = mesh.n_tets
+        self.n_field = mesh.n_nodes + mesh.n_edges
+        t2f = np.empty((10, mesh.n_tets), dtype=np.int64)
+        t2f[:4] = mesh.tets
+        t2f[4:] = mesh.tet_to_edge + mesh.n_nodes
+        self.tet_to_field = t2f
+        self.n_tet_dofs = 10

@@ -73,6 +73,7 @@ typedef struct {
 typedef struct em_ned2_plan_s* em_ned2_plan_t;
 typedef struct em_csc_map_s* em_csc_map_t;
 typedef struct em_graph_s* em_graph_t;
+typedef struct em_heat_plan_s* em_heat_plan_t;
 
 const char* em_last_error(void);
 const char* em_version(void);
@@ -310,6 +311,42 @@ int em_ned2_locate(em_ned2_plan_t plan, const double* coords, int64_t n, const i
 int em_ned2_interp(em_ned2_plan_t plan, const double* coords, int64_t n, const int64_t* tet_of,
                    const void* solution, int32_t which, const void* c_tet, void* out,
                    void* stream);
+
+/*
+ * Heat conduction, second-order Lagrange elements
+ * (physics/heatconduction/assembly/grad.py). A heat plan holds the node coordinates, the
+ * tets and the Legrange2 DOF map (tet_to_field int64 [10][T], elements/leg2.py:43-50:
+ * the 4 node DOFs then the 6 edge DOFs), copied to the device and checked once (ids in
+ * range; a zero-volume tet gives EM_EDEGENERATE).
+ */
+int em_heat_plan_create(em_heat_plan_t* plan, int device, const double* nodes, int64_t n_nodes,
+                        const int64_t* tets, int64_t n_tets, const int64_t* tet_to_field,
+                        int64_t n_field);
+int em_heat_plan_destroy(em_heat_plan_t plan);
+/*
+ * _stiff_matrix_builder / _mass_matrix_builder (grad.py:346-414) values: K = conductance
+ * blocks (leg2_tet_stiff, grad.py:199-261) from the conductivity cond (DEVICE float64,
+ * value (t, p*3+q) at cond[t * cond_tet_stride + (p*3+q) * cond_comp_stride] for
+ * EM_MAT_FULL; EM_MAT_DIAG reads 3 components, EM_MAT_SCALAR 1; the reference's (3,3,T)
+ * array is EM_MAT_FULL with strides 1, T) and M = capacity blocks (leg2_tet_mass,
+ * grad.py:265-338) from rho_cp[t * rho_tet_stride]; float64 [100 T] each, row-major 10x10
+ * per tet; either may be NULL. The reference's 11-point rule (its 8-digit table) is
+ * reproduced exactly, including its |V| normalisation on negatively oriented tets.
+ */
+int em_heat_assemble(em_heat_plan_t plan, const double* cond, int32_t cond_kind,
+                     int64_t cond_tet_stride, int64_t cond_comp_stride, const double* rho_cp,
+                     int64_t rho_tet_stride, double* K, double* M, void* stream);
+/* rows/cols int64 [100 T] of the blocks (grad.py:366-370). */
+int em_heat_rowcol(em_heat_plan_t plan, int64_t* rows, int64_t* cols, void* stream);
+/* CSCMapping.from_rowcol of the heat pattern (csc_cast.py:51-53, 65-160). Blocks. */
+int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream);
+/*
+ * CSCMapping.to_csc for float64 data (scatter_to_csc_f8, csc_cast.py:180-194):
+ * data_csc[q] = sum of data[k] (+ scale * data_b[k] when data_b is not NULL, e.g. a
+ * transient K + M / dt) over the entry's COO contributors, in COO order.
+ */
+int em_csc_to_csc_real(em_csc_map_t map, const double* data, const double* data_b, double scale,
+                       double* data_csc, void* stream);
 
 /*
  * HIP graphs of assembly launches. The reference re-runs its Numba builder per call

@@ -1,0 +1,126 @@
+"""GPU parity of the heat-conduction (Lagrange-2) assembly (heat.hip, emerge_amd.heat)
+against the oracle restatement of grad.py, the CSC pattern against SciPy, and a global
+energy pin of the assembled conductance matrix."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import heat as oh
+from tests.conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(which, wr90_field, delaunay_field):
+    from emerge_amd.mesh import Legrange2Field
+    return Legrange2Field((wr90_field if which == "wr90" else delaunay_field).mesh)
+
+
+def _cond(T, rng, kind):
+    C = np.zeros((3, 3, T))
+    if kind == "scalar":
+        v = rng.uniform(1, 5, T)
+        for d in range(3):
+            C[d, d] = v
+    elif kind == "diag":
+        for d in range(3):
+            C[d, d] = rng.uniform(1, 5, T)
+    else:
+        A = rng.normal(size=(3, 3, T)) * 0.3
+        C = A + np.eye(3)[:, :, None] * 3
+    return C
+
+
+@pytest.mark.parametrize("which", ["wr90", "delaunay"])
+@pytest.mark.parametrize("kind", ["scalar", "diag", "full"])
+def test_heat_blocks_match_oracle(which, kind, wr90_field, delaunay_field):
+    import torch
+    from emerge_amd.heat import HeatPlan
+    f = _field(which, wr90_field, delaunay_field)
+    m = f.mesh
+    T = f.n_tets
+    rng = np.random.default_rng(5)
+    C = _cond(T, rng, kind)
+    rho = rng.uniform(1e6, 4e6, T)
+    plan = HeatPlan.from_field(f)
+    if kind == "scalar":
+        cond = torch.as_tensor(C[0, 0].copy())
+    elif kind == "diag":
+        cond = torch.as_tensor(np.stack([C[d, d] for d in range(3)], axis=1).copy())
+    else:
+        cond = torch.as_tensor(C)  # reference (3, 3, T) layout
+    K, M = plan.assemble(cond=cond, rho_cp=torch.as_tensor(rho))
+    Kr, Mr, rows, cols = oh.build(m.nodes, m.tets, f.tet_to_field, cond=C, rho_cp=rho,
+                                  rowcol=True, local_frame=(which == "delaunay"))
+    assert rel_fro(K.cpu().numpy(), Kr) <= 1e-12
+    assert rel_fro(M.cpu().numpy(), Mr) <= 1e-12
+    r, c = plan.rowcol()
+    assert np.array_equal(r.cpu().numpy(), rows) and np.array_equal(c.cpu().numpy(), cols)
+
+
+def test_negative_orientation_tets(delaunay_field):
+    """Tets stored with negative orientation follow the reference's |V| normalisation."""
+    import torch
+    from emerge_amd.heat import HeatPlan
+    from emerge_amd.mesh import Legrange2Field, TetMesh
+    m0 = delaunay_field.mesh
+    tets = m0.tets.copy()
+    flip = np.arange(0, m0.n_tets, 3)
+    tets[[2, 3]][:, flip]
+    tets[2, flip], tets[3, flip] = m0.tets[3, flip], m0.tets[2, flip]
+    f = Legrange2Field(TetMesh(m0.nodes, tets))
+    assert np.sum(f.mesh.signed_dets() < 0) == len(flip)
+    T = f.n_tets
+    rng = np.random.default_rng(6)
+    C = _cond(T, rng, "full")
+    rho = rng.uniform(1, 2, T)
+    K, M = HeatPlan.from_field(f).assemble(cond=torch.as_tensor(C), rho_cp=torch.as_tensor(rho))
+    Kr, Mr, _, _ = oh.build(f.mesh.nodes, f.mesh.tets, f.tet_to_field, cond=C, rho_cp=rho,
+                            local_frame=True)
+    assert rel_fro(K.cpu().numpy(), Kr) <= 1e-12
+    assert rel_fro(M.cpu().numpy(), Mr) <= 1e-12
+
+
+def test_heat_csc_and_energy(wr90_field):
+    """CSC of the conductance equals SciPy's COO->CSC; a linear temperature u = g . x
+    (in the Lagrange-2 space) has energy u^T K u = c |g|^2 vol * sum(w) exactly, and K
+    annihilates constants."""
+    import torch
+    from emerge_amd.heat import HeatCscMap, HeatPlan, tet_mass_matrix, tet_stiffness_matrix
+    from emerge_amd.mesh import Legrange2Field
+    f = Legrange2Field(wr90_field.mesh)
+    m = f.mesh
+    T = f.n_tets
+    c = 2.5
+    plan = HeatPlan.from_field(f)
+    K, M = plan.assemble(cond=torch.tensor(c, dtype=torch.float64),
+                         rho_cp=torch.ones(T, dtype=torch.float64))
+    cm = HeatCscMap(plan)
+    r, cc = plan.rowcol()
+    S = sp.coo_matrix((K.cpu().numpy(), (r.cpu().numpy(), cc.cpu().numpy())),
+                      shape=(f.n_field, f.n_field)).tocsc()
+    S.sum_duplicates()
+    S.sort_indices()
+    ex = cm.export("indptr", "indices")
+    assert np.array_equal(ex["indptr"].cpu().numpy(), S.indptr)
+    assert np.array_equal(ex["indices"].cpu().numpy(), S.indices)
+    vals = cm.to_csc(K).cpu().numpy()
+    assert rel_fro(vals, S.data) <= 1e-14
+    A = sp.csc_matrix((vals, S.indices, S.indptr), shape=S.shape)
+    g = np.array([0.3, -1.2, 0.7])
+    xn = m.nodes
+    u = np.concatenate([g @ xn, g @ (0.5 * (xn[:, m.edges[0]] + xn[:, m.edges[1]]))])
+    vol = 22.86e-3 * 10.16e-3 * 50e-3
+    wsum = oh.REFERENCE_RULE[0].sum()
+    assert abs(u @ (A @ u) - c * (g @ g) * vol * wsum) <= 1e-10 * c * (g @ g) * vol
+    assert np.abs(A @ np.ones(f.n_field)).max() <= 1e-10 * np.abs(vals).max()
+    # transient combine K + M / dt in one pass
+    dt = 1e-3
+    both = cm.to_csc(K, M, 1 / dt).cpu().numpy()
+    assert rel_fro(both, vals + cm.to_csc(M).cpu().numpy() / dt) <= 1e-14
+    # drop-in entry points (grad.py:417-462)
+    Kd, mapping = tet_stiffness_matrix(f, np.full((3, 3, T), 0.0) + c * np.eye(3)[:, :, None])
+    assert rel_fro(Kd, K.cpu().numpy()) <= 1e-14
+    assert (mapping.to_csc(Kd) - A).nnz == 0 or sp.linalg.norm(mapping.to_csc(Kd) - A) <= 1e-12
+    Md, _ = tet_mass_matrix(f, np.ones(T))
+    assert rel_fro(Md, M.cpu().numpy()) <= 1e-14

@@ -378,6 +378,52 @@ def suite_surface_interp(args) -> dict:
                     "counts; interp oracle uses OpenMP threads"}
 
 
+def suite_heat(args) -> dict:
+    """Lagrange-2 heat assembly (conductance and capacity, FP64 COO) on a ~2.8M-tet mesh
+    (the bench mesh), its CSC map and the per-step to_csc(K + M / dt); CPU line: the
+    oracle's restatement of grad.py's builders on a bounded sample (16 threads)."""
+    import numpy as np
+    import torch
+    from oracle import heat as oh
+    from emerge_amd.heat import HeatCscMap, HeatPlan
+    from emerge_amd.mesh import Legrange2Field, delaunay_box_mesh
+    f = Legrange2Field(delaunay_box_mesh(args.points, seed=args.seed))
+    plan = HeatPlan.from_field(f)
+    T, Nn = f.n_tets, f.mesh.n_nodes
+    rng = np.random.default_rng(args.seed + 9)
+    cond = torch.as_tensor(rng.uniform(1, 400, (T, 3)), device="cuda")
+    rho = torch.as_tensor(rng.uniform(1e6, 4e6, T), device="cuda")
+    K, M = plan.assemble(cond=cond, rho_cp=rho)
+    ms, _ = time_launches(lambda: plan.assemble(cond=cond, rho_cp=rho), 10, 2)
+    # per tet: 16 ids + 24 Nn/T coordinates + 24 (diagonal conductivity) + 8 (rho) in,
+    # 2 x 800 out
+    b = 16 + 24 * Nn / T + 24 + 8 + 1600
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cm = HeatCscMap(plan)
+    torch.cuda.synchronize()
+    ms_map = (time.perf_counter() - t0) * 1e3
+    ms_csc, _ = time_launches(lambda: cm.to_csc(K, M, 1e3), 10, 2)
+    n = min(T, 100_000)
+    m = f.mesh
+    C = np.zeros((3, 3, n))
+    cnp = cond[:n].cpu().numpy()
+    for d in range(3):
+        C[d, d] = cnp[:, d]
+    t0 = time.perf_counter()
+    oh.build(m.nodes, m.tets[:, :n], f.tet_to_field[:, :n], cond=C, rho_cp=rho[:n].cpu().numpy())
+    cpu_tps = n / (time.perf_counter() - t0)
+    del K, M, cm
+    torch.cuda.empty_cache()
+    return {"n_tets": T, "n_dofs": f.n_field, "ms_assemble_K_and_M": ms,
+            "tets_per_s": T / (ms * 1e-3), "bytes_per_tet": b,
+            "gb_per_s": T * b / (ms * 1e-3) / 1e9,
+            "roofline_frac": T * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "ms_csc_map_one_time": ms_map, "ms_to_csc_K_plus_M_over_dt": ms_csc,
+            "cpu_oracle_tets_per_s": cpu_tps,
+            "cpu_sample": f"first {n} tets, K and M, OpenMP threads"}
+
+
 def target_run(args) -> dict:
     """The large single-mesh case: mixed and FP64 fused A on a ~8M-tet Delaunay mesh."""
     import numpy as np
@@ -442,7 +488,7 @@ def main() -> int:
                          "32-mesh batch and the backward on 1M tets")
     ap.add_argument("--only", default="",
                     help="with --suite: comma list among sweep,batch,backward,csc_pec,"
-                         "surface_interp")
+                         "surface_interp,heat")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -579,7 +625,8 @@ def main() -> int:
                     ("batch", "batch_32_meshes", suite_batch),
                     ("backward", "backward_1M", suite_backward),
                     ("csc_pec", "csc_pec_500k", suite_csc_pec),
-                    ("surface_interp", "surface_interp_500k", suite_surface_interp))
+                    ("surface_interp", "surface_interp_500k", suite_surface_interp),
+                    ("heat", "heat_2.8M", suite_heat))
             only = set(args.only.split(",")) if args.only else None
             for short, key, fn in runs:
                 if only is None or short in only:

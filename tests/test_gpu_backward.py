@@ -144,3 +144,51 @@ def test_full_tensor_leaves_and_mixed_forward(setup):
     ge32, gm32, _ = _grad(s, "mixed")
     assert np.linalg.norm(ge32 - ge64) <= 1e-6 * np.linalg.norm(ge64)
     assert np.linalg.norm(gm32 - gm64) <= 1e-6 * np.linalg.norm(gm64)
+
+
+def test_subset_plan_scalar_and_diag_leaves(setup):
+    """Conductor tets removed (a plan over a subset of 70% of the tets, not a multiple of
+    the kernel's 64-tet tiles), scalar and diagonal leaves, one leaf at a time: the
+    gradient of every assembled tet equals the linear-term reference and removed tets get
+    zero."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.autograd import assemble_a, bilinear_form
+    s = setup
+    f, T, k0 = s["field"], s["T"], s["k0"]
+    rng = np.random.default_rng(23)
+    keep = np.sort(rng.choice(T, size=int(0.7 * T), replace=False))
+    assert len(keep) % 64 != 0
+    plan = Ned2Plan.from_field(f, tetids=keep)
+    rows, cols = plan.rowcol()
+    lam, x = s["lam"], s["x"]
+    G = lam[rows] * torch.conj(x[cols])
+    ids = torch.as_tensor(keep, device="cuda")
+    eps = s["eps"].clone()
+    mu = s["muinv"].clone()
+    # scalar eps only
+    e = eps.clone().requires_grad_(True)
+    L = bilinear_form(assemble_a(plan, e, mu, k0), rows, cols, lam, x)
+    L.backward()
+    one = torch.ones(T, dtype=torch.complex128, device="cuda")
+    zero3 = torch.zeros((T, 3), dtype=torch.complex128, device="cuda")
+    dM = plan.assemble_a(plan.material(one), plan.material(zero3, inverse=True), k0, "fp64")
+    ref = torch.zeros(T, dtype=torch.complex128, device="cuda")
+    ref[ids] = (dM * G).reshape(-1, 400).sum(dim=1)
+    assert torch.linalg.norm(e.grad - ref) <= 1e-12 * torch.linalg.norm(ref)
+    removed = torch.ones(T, dtype=torch.bool, device="cuda")
+    removed[ids] = False
+    assert torch.all(e.grad[removed] == 0)
+    # diagonal mu^-1 only
+    m = mu.clone().requires_grad_(True)
+    L = bilinear_form(assemble_a(plan, eps, m, k0), rows, cols, lam, x)
+    L.backward()
+    zero = torch.zeros(T, dtype=torch.complex128, device="cuda")
+    for d in range(3):
+        unit = torch.zeros((T, 3), dtype=torch.complex128, device="cuda")
+        unit[:, d] = 1.0
+        dK = plan.assemble_a(plan.material(zero), plan.material(unit, inverse=True), k0, "fp64")
+        ref = torch.zeros(T, dtype=torch.complex128, device="cuda")
+        ref[ids] = (dK * G).reshape(-1, 400).sum(dim=1)
+        assert torch.linalg.norm(m.grad[:, d] - ref) <= 1e-12 * torch.linalg.norm(ref), d
+        assert torch.all(m.grad[removed, d] == 0)

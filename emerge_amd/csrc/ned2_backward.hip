@@ -265,29 +265,75 @@ __global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
 }
 
 // Scalar/diagonal leaves (no antisymmetric parts): one wavefront per 64-tet tile, no
-// barriers across wavefronts. Lanes 0..59 own one (component c of the 12, range of 42
-// pairs) each, with those 42 reference-tensor coefficients in registers (the M ones
-// times -k0^2). The upstream 20x20 blocks are read coalesced into registers kWAhead tets
-// ahead, staged in LDS a group at a time (the loads of the next group are in flight while
-// the current one is processed). Per tet every lane forms the signed weights of pairs
-// lane, lane + 64, ... into LDS, each owning lane sums its 42 weight x coefficient
-// products and five lanes reduce them by shuffles; the 12 sums of 16 tets are kept in LDS
-// and 16 lanes then apply the chain rule, one tet each (geometry recomputed there).
-constexpr int kWTile = 64;     // tets per wavefront tile
+// barriers across wavefronts. The wavefront's four 16-lane rows own three of the 12
+// components each (row r: components 3r .. 3r+2) and lane j of a row owns pairs
+// 14j .. 14j+13 (lane 15 none), with those 3 x 14 reference-tensor coefficients in
+// registers (the M ones times -k0^2); so one LDS read of a pair weight feeds six FMAs and
+// the 16 partial sums of a row are reduced by DPP lane moves (no LDS traffic). The
+// upstream 20x20 blocks are read coalesced into registers kWAhead tets ahead and staged
+// in LDS a group at a time (the loads of the next group are in flight while the current
+// one is processed). Per tet every lane forms the signed weights of pairs lane,
+// lane + 64, ... into LDS; the 12 sums of 16 tets are kept in LDS and 16 lanes then apply
+// the chain rule, one tet each.
+constexpr int kWTile = 64;     // tets per wavefront tile (one lane per tet for geometry)
 constexpr int kWChunk = 16;    // tets per chain-rule batch
 constexpr int kWWaves = 4;     // wavefronts per workgroup
-constexpr int kWSplit = 5;     // pair ranges per component
-constexpr int kWPairs = 42;    // pairs per range (5 x 42 = 210)
+constexpr int kWAhead = 1;     // upstream blocks staged (and prefetched) per group
+constexpr int kWComps = 3;     // components per lane (per 16-lane row)
+constexpr int kWPairs = 14;    // pairs per lane (15 x 14 = 210)
+constexpr int kWStride = 15;   // LDS slots per lane range (one pad: no bank conflicts)
+constexpr int kWSlots = 16 * kWStride;
 
-constexpr int kWAhead = 2;     // upstream blocks staged (and prefetched) per group
-
+// 13.3 KB per wavefront: two 4-wavefront workgroups fit a CU, two wavefronts per SIMD.
 struct WaveSmem {
     double2 blk[kWAhead][400];
-    double2 w[NED2_NPAIR];
+    double2 w[kWAhead][kWSlots];  // pair p at (p / 14) * 15 + p % 14; lane 15's range is zero
     double2 res[kWChunk][12];
-    int code[kWTile];
-    double sgn[kWTile];
 };
+
+// Cofactor matrix of J (row-major; C[3p + r]).
+__device__ __forceinline__ void cofactors(const double J[9], double C[9]) {
+    C[0] = J[4] * J[8] - J[5] * J[7];
+    C[1] = J[5] * J[6] - J[3] * J[8];
+    C[2] = J[3] * J[7] - J[4] * J[6];
+    C[3] = J[7] * J[2] - J[8] * J[1];
+    C[4] = J[8] * J[0] - J[6] * J[2];
+    C[5] = J[6] * J[1] - J[7] * J[0];
+    C[6] = J[1] * J[5] - J[2] * J[4];
+    C[7] = J[2] * J[3] - J[0] * J[5];
+    C[8] = J[0] * J[4] - J[1] * J[3];
+}
+
+// Diagonal of the pullback for a symmetric dL/dG given by its six components
+// (00, 11, 22, 01, 02, 12; complex, in LDS): with a_p = A[3p + r],
+//   dL/dX_rr = iad (w00 a0^2 + w11 a1^2 + w22 a2^2 + w01 a0 a1 + w02 a0 a2 + w12 a1 a2)
+// (the off-diagonal entries of dL/dG are half the split's components), reduced to a
+// scalar (trace) or diagonal leaf.
+__device__ __forceinline__ void write_diag_leaf(double2* out, long long t, int kind,
+                                                const double A[9], const double2* wsym,
+                                                double iad) {
+    double ore[3], oim[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double a0 = A[r], a1 = A[3 + r], a2 = A[6 + r];
+        const double sq[6] = {a0 * a0, a1 * a1, a2 * a2, a0 * a1, a0 * a2, a1 * a2};
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double2 v = wsym[c];
+            re += sq[c] * v.x;
+            im += sq[c] * v.y;
+        }
+        ore[r] = re * iad;
+        oim[r] = im * iad;
+    }
+    if (kind == EM_MAT_SCALAR) {
+        out[t] = make_double2(ore[0] + ore[1] + ore[2], oim[0] + oim[1] + oim[2]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) out[3 * t + r] = make_double2(ore[r], oim[r]);
+    }
+}
 
 // Geometry of assembly index ia: Jc (rows = columns of J, canonical vertex order), the
 // cofactor matrix C, 1/|det J|, tet id, vertex-rank code and orientation sign.
@@ -309,15 +355,7 @@ __device__ __forceinline__ void bwd_geometry(const MeshDev& m, long long ia, dou
     const double4 X0 = m.xyz[w0], X1 = m.xyz[w1], X2 = m.xyz[w2], X3 = m.xyz[w3];
     const double J[9] = {X1.x - X0.x, X1.y - X0.y, X1.z - X0.z, X2.x - X0.x, X2.y - X0.y,
                          X2.z - X0.z, X3.x - X0.x, X3.y - X0.y, X3.z - X0.z};
-    C[0] = J[4] * J[8] - J[5] * J[7];
-    C[1] = J[5] * J[6] - J[3] * J[8];
-    C[2] = J[3] * J[7] - J[4] * J[6];
-    C[3] = J[7] * J[2] - J[8] * J[1];
-    C[4] = J[8] * J[0] - J[6] * J[2];
-    C[5] = J[6] * J[1] - J[7] * J[0];
-    C[6] = J[1] * J[5] - J[2] * J[4];
-    C[7] = J[2] * J[3] - J[0] * J[5];
-    C[8] = J[0] * J[4] - J[1] * J[3];
+    cofactors(J, C);
 #pragma unroll
     for (int i = 0; i < 9; ++i) Jc[i] = J[i];
     const double det = J[0] * C[0] + J[1] * C[1] + J[2] * C[2];
@@ -334,88 +372,116 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One tet of a wavefront tile whose upstream block is staged in blk: the signed pair
-// weights and the 12 sums. When a 16-tet chunk is complete its chain rule runs, one lane
-// per tet.
-__device__ __forceinline__ void wave_tet(const MeshDev& m, WaveSmem& sm, const double2* blk,
-                                         const unsigned char (*slot)[20],
-                                         const double (&coef)[kWPairs], const int (&pa)[4],
-                                         const int (&pb)[4], int lane, bool owner, int comp,
-                                         int split, long long tile0, int lt, int ntile,
-                                         int eps_kind, int mu_kind, double2* __restrict__ g_eps,
-                                         double2* __restrict__ g_mu) {
-    const int code = sm.code[lt];
-    const double s = sm.sgn[lt];
+// x moved across lanes by the DPP control CTRL (a bitwise move of both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of each row, returned in every lane of the row: lane ^ 1, lane ^ 2
+// (quad_perm), then the 8-lane half mirror and the 16-lane row mirror.
+__device__ __forceinline__ double row_sum(double x) {
+    x += dpp_move<0xb1>(x);
+    x += dpp_move<0x4e>(x);
+    x += dpp_move<0x141>(x);
+    x += dpp_move<0x140>(x);
+    return x;
+}
+
+// Geometry of the lane's own tet of the tile (lane t holds tet tile0 + t).
+struct LaneGeom {
+    double Jc[9], iad;
+    long long tet;
+    int code, neg;
+};
+
+// One group of nq <= kWAhead tets of a wavefront tile (lt0 ..), their upstream blocks
+// staged in sm.blk: the signed pair weights and the 12 sums of every tet of the group.
+// When a 16-tet chunk is complete its chain rule runs on the lanes that hold the chunk's
+// geometry.
+__device__ __forceinline__ void wave_group(WaveSmem& sm, const unsigned char (*slot)[20],
+                                           const double (&coef)[kWComps][kWPairs],
+                                           const int (&pa)[4], const int (&pb)[4],
+                                           const int (&pw)[4], const LaneGeom& gm, int lane,
+                                           int row, int j, int lt0, int nq, int ntile,
+                                           int eps_kind, int mu_kind,
+                                           double2* __restrict__ g_eps,
+                                           double2* __restrict__ g_mu) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int p = lane + 64 * k;
-        if (p < NED2_NPAIR) {
-            const int a = pa[k], b = pb[k];
-            const int ra = slot[code][a], rb = slot[code][b];
-            double2 v = blk[ra * 20 + rb];
-            if (ra != rb) {
-                const double2 v2 = blk[rb * 20 + ra];
-                v.x += v2.x;
-                v.y += v2.y;
+    for (int q = 0; q < kWAhead; ++q) {
+        const int lt = min(lt0 + q, ntile - 1);
+        const int code = __builtin_amdgcn_readlane(gm.code, lt);
+        const double s = __builtin_amdgcn_readlane(gm.neg, lt) ? -1.0 : 1.0;
+        const double2* blk = sm.blk[q];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = lane + 64 * k;
+            if (p < NED2_NPAIR) {
+                const int a = pa[k], b = pb[k];
+                const int ra = slot[code][a], rb = slot[code][b];
+                double2 v = blk[ra * 20 + rb];
+                if (ra != rb) {
+                    const double2 v2 = blk[rb * 20 + ra];
+                    v.x += v2.x;
+                    v.y += v2.y;
+                }
+                const double sgp = (a >= 6 ? s : 1.0) * (b >= 6 ? s : 1.0);
+                sm.w[q][pw[k]] = make_double2(sgp * v.x, sgp * v.y);
             }
-            const double sgp = (a >= 6 ? s : 1.0) * (b >= 6 ? s : 1.0);
-            sm.w[p] = make_double2(sgp * v.x, sgp * v.y);
         }
     }
     wave_sync();
-    double re = 0.0, im = 0.0;
-    if (owner) {
-        const double2* wp = sm.w + split * kWPairs;
+    double acc[kWAhead][kWComps][2];
 #pragma unroll
-        for (int k = 0; k < kWPairs; ++k) {
-            const double2 v = wp[k];
-            re += v.x * coef[k];
-            im += v.y * coef[k];
+    for (int q = 0; q < kWAhead; ++q)
+#pragma unroll
+        for (int c = 0; c < kWComps; ++c) acc[q][c][0] = acc[q][c][1] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kWPairs; ++k) {
+#pragma unroll
+        for (int q = 0; q < kWAhead; ++q) {
+            const double2 v = sm.w[q][j * kWStride + k];
+#pragma unroll
+            for (int c = 0; c < kWComps; ++c) {
+                acc[q][c][0] += v.x * coef[c][k];
+                acc[q][c][1] += v.y * coef[c][k];
+            }
         }
     }
-    // sum over the 5 ranges of a component: lanes 5c .. 5c+4
-    double rs = re, is = im;
 #pragma unroll
-    for (int d = 1; d < kWSplit; ++d) {
-        rs += __shfl_down(re, d);
-        is += __shfl_down(im, d);
+    for (int q = 0; q < kWAhead; ++q) {
+        double2 mine = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < kWComps; ++c) {
+            const double re = row_sum(acc[q][c][0]), im = row_sum(acc[q][c][1]);
+            if (j == c) mine = make_double2(re, im);
+        }
+        if (q < nq && j < kWComps) sm.res[(lt0 + q) % kWChunk][kWComps * row + j] = mine;
     }
-    const int u = lt % kWChunk;
-    if (owner && split == 0) sm.res[u][comp] = make_double2(rs, is);
     wave_sync();
-    if (u != kWChunk - 1 && lt != ntile - 1) return;
-    // chain rule for the chunk's tets, one lane each (geometry recomputed)
-    const int c0 = lt - u;
-    if (lane <= u) {
-        double Jc[9], C[9], iad, sg;
-        long long tet;
-        int cd;
-        bwd_geometry(m, tile0 + c0 + lane, Jc, C, iad, tet, cd, sg);
-        double w[24];
-#pragma unroll
-        for (int c = 0; c < 12; ++c) {
-            const double2 v = sm.res[lane][c];
-            w[2 * c] = v.x;
-            w[2 * c + 1] = v.y;
-        }
-        double Wre[9], Wim[9], Ore[9], Oim[9];
-        if (g_mu) {
-            unsplit(w, w, false, Wre, Wim);
-            pullback(Jc, Wre, Wim, iad, Ore, Oim);
-            write_leaf(g_mu, tet, mu_kind, Ore, Oim);
-        }
+    const int last = lt0 + nq - 1;
+    const int u = last % kWChunk;
+    if (u != kWChunk - 1 && last != ntile - 1) return;
+    // chain rule for the chunk's tets on the lanes that hold them
+    const int c0 = last - u;
+    if (lane >= c0 && lane <= last) {
+        const double2* r = sm.res[lane - c0];
+        if (g_mu) write_diag_leaf(g_mu, gm.tet, mu_kind, gm.Jc, r, gm.iad);
         if (g_eps) {
-            unsplit(w + 12, w + 12, false, Wre, Wim);
-            pullback(C, Wre, Wim, iad, Ore, Oim);
-            write_leaf(g_eps, tet, eps_kind, Ore, Oim);
+            double C[9];
+            cofactors(gm.Jc, C);
+            write_diag_leaf(g_eps, gm.tet, eps_kind, C, r + 6, gm.iad);
         }
     }
     wave_sync();
 }
 
-__global__ __launch_bounds__(64 * kWWaves) void ned2_backward_wave_kernel(
-    MeshDev m, const double2* __restrict__ grad, double k0sq, int eps_kind, int mu_kind,
-    double2* __restrict__ g_eps, double2* __restrict__ g_mu) {
+__global__ __launch_bounds__(64 * kWWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, double k0sq,
+                               int eps_kind, int mu_kind, double2* __restrict__ g_eps,
+                               double2* __restrict__ g_mu) {
     __shared__ WaveSmem smem[kWWaves];
     __shared__ unsigned char slot[64][20];
     for (int i = threadIdx.x; i < 64 * 20; i += 64 * kWWaves)
@@ -438,30 +504,37 @@ __global__ __launch_bounds__(64 * kWWaves) void ned2_backward_wave_kernel(
             bre[q][k] = v.x;
             bim[q][k] = v.y;
         }
-    // this lane's (component, pair range) and coefficients, and its pairs' functions
-    const bool owner = lane < 12 * kWSplit;
-    const int comp = owner ? lane / kWSplit : 0;
-    const int split = owner ? lane % kWSplit : 0;
-    double coef[kWPairs];
+    // this lane's components and pair range with their coefficients, and the functions
+    // and LDS slots of the pairs whose weights it forms
+    const int row = lane >> 4, j = lane & 15;
+    double coef[kWComps][kWPairs];
 #pragma unroll
-    for (int k = 0; k < kWPairs; ++k) {
-        const int p = split * kWPairs + k;
-        coef[k] = comp < 6 ? TSYM_K[p][comp] : -k0sq * TSYM_M[p][comp - 6];
-    }
-    int pa[4], pb[4];
+    for (int c = 0; c < kWComps; ++c)
+#pragma unroll
+        for (int k = 0; k < kWPairs; ++k) {
+            const int comp = kWComps * row + c;
+            const int p = min(kWPairs * j + k, NED2_NPAIR - 1);
+            const double v = comp < 6 ? TSYM_K[p][comp] : -k0sq * TSYM_M[p][comp - 6];
+            coef[c][k] = j < 15 ? v : 0.0;
+        }
+    int pa[4], pb[4], pw[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int p = min(lane + 64 * k, NED2_NPAIR - 1);
         pa[k] = PAIR_A[p];
         pb[k] = PAIR_B[p];
+        pw[k] = (p / kWPairs) * kWStride + p % kWPairs;
     }
-    if (lane < ntile) {
-        double Jc[9], C[9], iad, sg;
-        long long tet;
-        int code;
-        bwd_geometry(m, tile0 + lane, Jc, C, iad, tet, code, sg);
-        sm.sgn[lane] = sg;
-        sm.code[lane] = code;
+    // lane 15's range (read with zero coefficients) is never written by the pair phase
+#pragma unroll
+    for (int q = 0; q < kWAhead; ++q)
+        if (lane < kWStride) sm.w[q][15 * kWStride + lane] = make_double2(0.0, 0.0);
+    // geometry of the lane's tet
+    LaneGeom gm;
+    {
+        double C[9], sg;
+        bwd_geometry(m, tile0 + min(lane, ntile - 1), gm.Jc, C, gm.iad, gm.tet, gm.code, sg);
+        gm.neg = sg < 0.0;
     }
     for (int lt0 = 0; lt0 < ntile; lt0 += kWAhead) {
         // stage the blocks, then start the loads of the next group into the same registers
@@ -484,10 +557,8 @@ __global__ __launch_bounds__(64 * kWWaves) void ned2_backward_wave_kernel(
                 }
             }
         wave_sync();
-        const int nq = min(kWAhead, ntile - lt0);
-        for (int q = 0; q < nq; ++q)
-            wave_tet(m, sm, sm.blk[q], slot, coef, pa, pb, lane, owner, comp, split, tile0,
-                     lt0 + q, ntile, eps_kind, mu_kind, g_eps, g_mu);
+        wave_group(sm, slot, coef, pa, pb, pw, gm, lane, row, j, lt0,
+                   min(kWAhead, ntile - lt0), ntile, eps_kind, mu_kind, g_eps, g_mu);
     }
 }
 

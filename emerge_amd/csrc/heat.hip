@@ -7,8 +7,10 @@
 // applied to the polynomials (gen_heat_tables.py), so per tet
 //   K_ij = V sum_ab (grad L_b . C grad L_a) HEAT_Q[s][ij][ab],  M_ij = rho_cp V HEAT_R[s][ij]
 // with V = |det J| / 6, grad L_a the barycentric gradients (from vertex differences) and s
-// the sign of det J (the reference's |V| normalisation). HBM-bound: 1,600 B written per tet
-// against ~100 B read; the contraction is 16 FMA per K entry.
+// the sign of det J (the reference's |V| normalisation). For a symmetric conductivity
+// (scalar, diagonal) H is symmetric and the sum runs over its 10 distinct terms with the
+// off-diagonal coefficient pairs added. HBM-bound: 1,600 B written per tet against
+// ~100 B read; the contraction is 10 (16) FMA per K entry.
 #include <hip/hip_runtime.h>
 
 #include "em_internal.h"
@@ -18,37 +20,63 @@ namespace em {
 
 namespace {
 
-constexpr int kHeatBlock = 128;
+constexpr int kHeatBlock = 256;
 constexpr int kHeatTile = 64;
+constexpr int kHeatOwners = 250;  // 50 entry pairs x 5 tets per round
 
-struct HeatRec {
-    double H[kHeatTile][16];  // grad L_b . C grad L_a, index 4 a + b
-    double V[kHeatTile];
-    double rV[kHeatTile];
-    int s[kHeatTile];
+// Per tet of a tile: the conductance terms (16, or 10 when C is symmetric), V, rho_cp V and
+// the orientation, one 160-byte row.
+struct alignas(16) HeatRec {
+    double row[kHeatTile][20];  // terms [0, 16), V [16], rho_cp V [17], s [18]
+    int uniform;                // every tet of the tile has the orientation of tet 0
 };
 
+// Coefficients of entries e and e + 1 for orientation s: the 16 HEAT_Q terms, or for a
+// symmetric conductivity the 10 terms of the symmetric H (off-diagonal pairs summed).
+template <int NT>
+__device__ __forceinline__ void heat_coefs(int s, int e, double (&q)[2][NT]) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const double* Q = HEAT_Q[s][e + u];
+        if (NT == 16) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) q[u][k] = Q[k];
+        } else {
+            // H index 4 a + b; symmetric order 00 11 22 33 01 02 03 12 13 23
+            q[u][0] = Q[0];
+            q[u][1] = Q[5];
+            q[u][2] = Q[10];
+            q[u][3] = Q[15];
+            q[u][4] = Q[1] + Q[4];
+            q[u][5] = Q[2] + Q[8];
+            q[u][6] = Q[3] + Q[12];
+            q[u][7] = Q[6] + Q[9];
+            q[u][8] = Q[7] + Q[13];
+            q[u][9] = Q[11] + Q[14];
+        }
+    }
+}
+
+// NC conductivity values per tet (0: K not asked for, 1 scalar, 3 diagonal, 9 full); NT
+// conductance terms per tet (10 when C is symmetric, else 16).
+// Wave 0 computes the 64 tets' rows into LDS (one lane per tet). Then thread j < 250 owns
+// the entry pair (e, e + 1), e = 2 (j % 50), with its coefficients in registers, and
+// forms both entries of K and M for tets j / 50, j / 50 + 5, ...: consecutive threads
+// write consecutive 16-byte pairs, so every store instruction is one contiguous 1 KB run
+// of the tile's COO values.
+template <int NC>
 __global__ __launch_bounds__(kHeatBlock) void heat_assemble_kernel(
     const double4* __restrict__ xyz, const int4* __restrict__ tets, long long T,
-    const double* __restrict__ cond, int cond_kind, long long cts, long long ccs,
+    const double* __restrict__ cond, long long cts, long long ccs,
     const double* __restrict__ rho, long long rts, double* __restrict__ K,
     double* __restrict__ M) {
+    constexpr int NT = NC == 9 ? 16 : 10;
     __shared__ HeatRec rec;
     const int tid = threadIdx.x;
-    // coefficients of this thread's entry for both orientations
-    double q0[16], q1[16], r0 = 0.0, r1 = 0.0;
-    const int e = tid < 100 ? tid : 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        q0[k] = HEAT_Q[0][e][k];
-        q1[k] = HEAT_Q[1][e][k];
-    }
-    r0 = HEAT_R[0][e];
-    r1 = HEAT_R[1][e];
-    const long long ntiles = (T + kHeatTile - 1) / kHeatTile;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const long long t0 = tile * kHeatTile;
-        const int cnt = (int)(T - t0 < kHeatTile ? T - t0 : kHeatTile);
+    const long long t0 = (long long)blockIdx.x * kHeatTile;
+    const int cnt = (int)(T - t0 < kHeatTile ? T - t0 : kHeatTile);
+    if (tid < kHeatTile) {
+        int sg = 0;
         if (tid < cnt) {
             const long long t = t0 + tid;
             const int4 v = tets[t];
@@ -69,49 +97,128 @@ __global__ __launch_bounds__(kHeatBlock) void heat_assemble_kernel(
             for (int k = 1; k < 4; ++k)
                 for (int c = 0; c < 3; ++c) g[k][c] /= det;
             for (int c = 0; c < 3; ++c) g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
-            // the conductivity is read only when K is asked for (cond may be null otherwise)
-            double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (K) {
+            double* r = rec.row[tid];
+            if (NC > 0) {
                 const double* cp = cond + t * cts;
-                if (cond_kind == EM_MAT_SCALAR) {
+                double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (NC == 1) {
                     C[0] = C[4] = C[8] = cp[0];
-                } else if (cond_kind == EM_MAT_DIAG) {
+                } else if (NC == 3) {
                     C[0] = cp[0];
                     C[4] = cp[ccs];
                     C[8] = cp[2 * ccs];
                 } else {
+#pragma unroll
                     for (int k = 0; k < 9; ++k) C[k] = cp[k * ccs];
                 }
-            }
-            for (int a = 0; a < 4; ++a) {
-                const double cg[3] = {C[0] * g[a][0] + C[1] * g[a][1] + C[2] * g[a][2],
-                                      C[3] * g[a][0] + C[4] * g[a][1] + C[5] * g[a][2],
-                                      C[6] * g[a][0] + C[7] * g[a][1] + C[8] * g[a][2]};
-                for (int b = 0; b < 4; ++b)
-                    rec.H[tid][4 * a + b] = g[b][0] * cg[0] + g[b][1] * cg[1] + g[b][2] * cg[2];
+                // H_ab = grad L_b . C grad L_a
+                double H[16];
+                for (int a = 0; a < 4; ++a) {
+                    const double cg[3] = {C[0] * g[a][0] + C[1] * g[a][1] + C[2] * g[a][2],
+                                          C[3] * g[a][0] + C[4] * g[a][1] + C[5] * g[a][2],
+                                          C[6] * g[a][0] + C[7] * g[a][1] + C[8] * g[a][2]};
+                    for (int b = 0; b < 4; ++b)
+                        H[4 * a + b] = g[b][0] * cg[0] + g[b][1] * cg[1] + g[b][2] * cg[2];
+                }
+                if (NT == 16) {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) r[k] = H[k];
+                } else {
+                    r[0] = H[0];
+                    r[1] = H[5];
+                    r[2] = H[10];
+                    r[3] = H[15];
+                    r[4] = H[1];
+                    r[5] = H[2];
+                    r[6] = H[3];
+                    r[7] = H[6];
+                    r[8] = H[7];
+                    r[9] = H[11];
+                }
             }
             const double V = fabs(det) / 6.0;
-            rec.V[tid] = V;
-            rec.rV[tid] = rho ? rho[t * rts] * V : 0.0;
-            rec.s[tid] = det > 0 ? 0 : 1;
+            r[16] = V;
+            r[17] = rho ? rho[t * rts] * V : 0.0;
+            sg = det > 0 ? 0 : 1;
+            r[18] = (double)sg;
         }
-        __syncthreads();
-        if (tid < 100) {
-            for (int lt = 0; lt < cnt; ++lt) {
-                const long long o = (t0 + lt) * 100 + tid;
-                const bool neg = rec.s[lt] != 0;
-                if (K) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) acc += rec.H[lt][k] * (neg ? q1[k] : q0[k]);
-                    __builtin_nontemporal_store(acc * rec.V[lt], K + o);
-                }
-                if (M) __builtin_nontemporal_store(rec.rV[lt] * (neg ? r1 : r0), M + o);
+        const int s0 = __shfl(sg, 0);
+        const bool same = __ballot(tid < cnt && sg != s0) == 0;
+        if (tid == 0) rec.uniform = same ? 1 : 0;
+    }
+    __syncthreads();
+    if (tid >= kHeatOwners) return;
+    const int e = 2 * (tid % 50);
+    const int lt0 = tid / 50;
+    int sq = (int)rec.row[0][18];  // orientation whose coefficients are loaded
+    double q[2][NT];
+    if (NC > 0) heat_coefs<NT>(sq, e, q);
+    double R[2] = {HEAT_R[sq][e], HEAT_R[sq][e + 1]};
+    const bool uniform = rec.uniform != 0;
+    for (int lt = lt0; lt < cnt; lt += 5) {
+        const double* r = rec.row[lt];
+        if (!uniform) {
+            const int sl = (int)r[18];
+            if (sl != sq) {
+                sq = sl;
+                if (NC > 0) heat_coefs<NT>(sq, e, q);
+                R[0] = HEAT_R[sq][e];
+                R[1] = HEAT_R[sq][e + 1];
             }
         }
-        __syncthreads();
+        const long long o = (t0 + lt) * 100 + e;
+        if (NC > 0) {
+            double h[NT];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) h[k] = r[k];
+            double k0a = 0.0, k0b = 0.0, k1a = 0.0, k1b = 0.0;
+#pragma unroll
+            for (int k = 0; k < NT; k += 2) {
+                k0a = fma(h[k], q[0][k], k0a);
+                k1a = fma(h[k], q[1][k], k1a);
+                if (k + 1 < NT) {
+                    k0b = fma(h[k + 1], q[0][k + 1], k0b);
+                    k1b = fma(h[k + 1], q[1][k + 1], k1b);
+                }
+            }
+            const double V = r[16];
+            __builtin_nontemporal_store((k0a + k0b) * V, K + o);
+            __builtin_nontemporal_store((k1a + k1b) * V, K + o + 1);
+        }
+        if (M) {
+            const double rv = r[17];
+            __builtin_nontemporal_store(rv * R[0], M + o);
+            __builtin_nontemporal_store(rv * R[1], M + o + 1);
+        }
     }
 }
+
+}  // namespace
+
+hipError_t heat_assemble(const double4* xyz, const int4* tets, long long T, const double* cond,
+                         int cond_kind, long long cts, long long ccs, const double* rho,
+                         long long rts, double* K, double* M, hipStream_t s) {
+    if (T <= 0) return hipSuccess;
+    const long long ntiles = (T + kHeatTile - 1) / kHeatTile;
+    if (ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 g((unsigned)ntiles);
+    if (!K) {
+        hipLaunchKernelGGL(heat_assemble_kernel<0>, g, dim3(kHeatBlock), 0, s, xyz, tets, T,
+                           nullptr, 0LL, 0LL, rho, rts, nullptr, M);
+    } else if (cond_kind == EM_MAT_SCALAR) {
+        hipLaunchKernelGGL(heat_assemble_kernel<1>, g, dim3(kHeatBlock), 0, s, xyz, tets, T,
+                           cond, cts, ccs, rho, rts, K, M);
+    } else if (cond_kind == EM_MAT_DIAG) {
+        hipLaunchKernelGGL(heat_assemble_kernel<3>, g, dim3(kHeatBlock), 0, s, xyz, tets, T,
+                           cond, cts, ccs, rho, rts, K, M);
+    } else {
+        hipLaunchKernelGGL(heat_assemble_kernel<9>, g, dim3(kHeatBlock), 0, s, xyz, tets, T,
+                           cond, cts, ccs, rho, rts, K, M);
+    }
+    return hipGetLastError();
+}
+
+namespace {
 
 __global__ __launch_bounds__(256) void heat_rowcol_kernel(const int* __restrict__ t2f, long long T,
                                                           long long* __restrict__ rows,
@@ -126,17 +233,6 @@ __global__ __launch_bounds__(256) void heat_rowcol_kernel(const int* __restrict_
 }
 
 }  // namespace
-
-hipError_t heat_assemble(const double4* xyz, const int4* tets, long long T, const double* cond,
-                         int cond_kind, long long cts, long long ccs, const double* rho,
-                         long long rts, double* K, double* M, hipStream_t s) {
-    if (T <= 0) return hipSuccess;
-    long long ntiles = (T + kHeatTile - 1) / kHeatTile;
-    const unsigned g = (unsigned)(ntiles < 65536 * 4 ? ntiles : 65536 * 4);
-    hipLaunchKernelGGL(heat_assemble_kernel, dim3(g), dim3(kHeatBlock), 0, s, xyz, tets, T, cond,
-                       cond_kind, cts, ccs, rho, rts, K, M);
-    return hipGetLastError();
-}
 
 hipError_t heat_rowcol(const int* t2f, long long T, long long* rows, long long* cols,
                        hipStream_t s) {

@@ -58,15 +58,17 @@ def test_heat_blocks_match_oracle(which, kind, wr90_field, delaunay_field):
     assert np.array_equal(r.cpu().numpy(), rows) and np.array_equal(c.cpu().numpy(), cols)
 
 
-def test_negative_orientation_tets(delaunay_field):
-    """Tets stored with negative orientation follow the reference's |V| normalisation."""
+@pytest.mark.parametrize("every", [3, 1])
+def test_negative_orientation_tets(delaunay_field, every):
+    """Tets stored with negative orientation follow the reference's |V| normalisation:
+    every third tet flipped (tiles of mixed orientation) and every tet flipped (tiles of
+    one negative orientation)."""
     import torch
     from emerge_amd.heat import HeatPlan
     from emerge_amd.mesh import Legrange2Field, TetMesh
     m0 = delaunay_field.mesh
     tets = m0.tets.copy()
-    flip = np.arange(0, m0.n_tets, 3)
-    tets[[2, 3]][:, flip]
+    flip = np.arange(0, m0.n_tets, every)
     tets[2, flip], tets[3, flip] = m0.tets[3, flip], m0.tets[2, flip]
     f = Legrange2Field(TetMesh(m0.nodes, tets))
     assert np.sum(f.mesh.signed_dets() < 0) == len(flip)

@@ -403,7 +403,13 @@ def suite_heat(args) -> dict:
     cm = HeatCscMap(plan)
     torch.cuda.synchronize()
     ms_map = (time.perf_counter() - t0) * 1e3
-    ms_csc, _ = time_launches(lambda: cm.to_csc(K, M, 1e3), 10, 2)
+    # diagonal conductivity: every block is bit-symmetric, so the mirrored reads apply
+    ms_csc, _ = time_launches(lambda: cm.to_csc(K, M, 1e3, block_symmetric=True), 10, 2)
+    ms_csc_plain, _ = time_launches(lambda: cm.to_csc(K, M, 1e3), 10, 2)
+    ms_csc_seg, _ = time_launches(lambda: cm.to_csc(K, M, 1e3, segment_gather=True), 10, 2)
+    ref = cm.to_csc(K, M, 1e3, segment_gather=True)
+    same = bool(torch.equal(cm.to_csc(K, M, 1e3), ref)) and \
+        bool(torch.equal(cm.to_csc(K, M, 1e3, block_symmetric=True), ref))
     n = min(T, 100_000)
     m = f.mesh
     C = np.zeros((3, 3, n))
@@ -420,6 +426,9 @@ def suite_heat(args) -> dict:
             "gb_per_s": T * b / (ms * 1e-3) / 1e9,
             "roofline_frac": T * b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "ms_csc_map_one_time": ms_map, "ms_to_csc_K_plus_M_over_dt": ms_csc,
+            "ms_to_csc_K_plus_M_over_dt_plain_incidence": ms_csc_plain,
+            "ms_to_csc_K_plus_M_over_dt_segment_gather": ms_csc_seg,
+            "incidence_equals_segment_gather": same,
             "cpu_oracle_tets_per_s": cpu_tps,
             "cpu_sample": f"first {n} tets, K and M, OpenMP threads"}
 

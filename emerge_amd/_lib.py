@@ -115,7 +115,7 @@ _SIGS = {
     "em_heat_assemble": (ctypes.c_int, [P, P, I32, I64, I64, P, I64, P, P, P]),
     "em_heat_rowcol": (ctypes.c_int, [P, P, P, P]),
     "em_csc_map_create_heat": (ctypes.c_int, [ctypes.POINTER(P), P, P]),
-    "em_csc_to_csc_real": (ctypes.c_int, [P, P, P, ctypes.c_double, P, P]),
+    "em_csc_to_csc_real": (ctypes.c_int, [P, P, P, ctypes.c_double, P, ctypes.c_int, P]),
     "em_graph_begin_capture": (ctypes.c_int, [P]),
     "em_graph_end_capture": (ctypes.c_int, [P, ctypes.POINTER(P)]),
     "em_graph_launch": (ctypes.c_int, [P, P]),

@@ -1144,14 +1144,28 @@ int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream)
 }
 
 int em_csc_to_csc_real(em_csc_map_t map, const double* data, const double* data_b, double scale,
-                       double* data_csc, void* stream) {
+                       double* data_csc, int flags, void* stream) {
     if (!map || (map->b.nnz > 0 && (!data || !data_csc))) {
         set_error("to_csc_real: bad argument");
         return EM_EINVAL;
     }
+    if (flags & ~(EM_CSC_SEGMENT_GATHER | EM_CSC_BLOCK_SYMMETRIC)) {
+        set_error("to_csc_real: unknown flags");
+        return EM_EINVAL;
+    }
+    if ((flags & EM_CSC_SEGMENT_GATHER) && (flags & EM_CSC_BLOCK_SYMMETRIC)) {
+        set_error("to_csc_real: EM_CSC_BLOCK_SYMMETRIC applies to the incidence gather only");
+        return EM_EINVAL;
+    }
     DeviceGuard g(map->device);
-    EM_HIP(em::csc_gather_real(map->b.seg, map->b.perm, map->b.nnz, data, data_b, scale, data_csc,
-                               (hipStream_t)stream));
+    if (map->b.ndof == 10 && map->b.inc && map->b.pos && !(flags & EM_CSC_SEGMENT_GATHER)) {
+        EM_HIP(em::csc_gather_inc_real(map->b, map->N, data, data_b, scale,
+                                       (flags & EM_CSC_BLOCK_SYMMETRIC) != 0, data_csc,
+                                       (hipStream_t)stream));
+    } else {
+        EM_HIP(em::csc_gather_real(map->b.seg, map->b.perm, map->b.nnz, data, data_b, scale,
+                                   data_csc, (hipStream_t)stream));
+    }
     return EM_OK;
 }
 

@@ -381,7 +381,7 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
         const int* tetids;
         long long n_assy;
     } m = {t2f, tetids, n_assy};
-    constexpr bool kPos = NDOF == 20;
+    constexpr bool kPos = true;
     const long long nnz_coo = m.n_assy * NDOF * NDOF;
     hipError_t e;
     unsigned long long *cnt = nullptr, *maxlen = nullptr;
@@ -490,6 +490,81 @@ hipError_t csc_build_ndof10(const int* t2f, long long n_tets, long long N, CscBu
     hipError_t e = csc_build_n<10>(t2f, nullptr, n_tets, N, b, max_inc, s);
     if (e == hipSuccess) b.ndof = 10;
     return e;
+}
+
+// Real-valued data, column-incidence form (as csc_gather_inc_kernel): one group of NDOF
+// lanes per column, 64 / NDOF columns per wavefront; lane ii of a group reads entry
+// (ii, jj) of every incident block and adds it at its row position in an LDS column, in
+// incidence (= COO) order, so the result equals the segment gather's bit for bit.
+// MIRROR (bit-symmetric blocks): entry (ii, jj) is read from (jj, ii), so a group's reads
+// are one contiguous NDOF-entry row of each incident block.
+template <int NDOF, bool MIRROR, bool HAS_B>
+__global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_real_kernel(
+    const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
+    const int* __restrict__ inc, const unsigned short* __restrict__ pos, long long N,
+    const double* __restrict__ E, const double* __restrict__ B, double scale,
+    double* __restrict__ out) {
+    constexpr int kCols = 64 / NDOF;
+    __shared__ double acc[kIncWaves * kCols][kIncCap];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = lane / NDOF, ii = lane - NDOF * grp;
+    if (grp >= kCols) return;  // idle lanes: no barriers below
+    double* a = acc[wave * kCols + grp];
+    const long long stride = (long long)gridDim.x * kIncWaves * kCols;
+    for (long long c = ((long long)blockIdx.x * kIncWaves + wave) * kCols + grp; c < N;
+         c += stride) {
+        const long long cb = indptr[c], len = indptr[c + 1] - cb;
+        const long long s = inc_ptr[c], ninc = inc_ptr[c + 1] - s;
+        for (long long w0 = 0; w0 < len; w0 += kIncCap) {
+            const int wl = (int)(len - w0 < kIncCap ? len - w0 : kIncCap);
+            for (int p = ii; p < wl; p += NDOF) a[p] = 0.0;
+            constexpr int kU = 4;
+            for (long long i0 = 0; i0 < ninc; i0 += kU) {
+                int p[kU];
+                double v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const long long i = i0 + u < ninc ? i0 + u : ninc - 1;  // in range
+                    const int e = inc[s + i];
+                    p[u] = i0 + u < ninc ? (int)pos[NDOF * (s + i) + ii] - (int)w0 : -1;
+                    const long long ia = e / NDOF;
+                    const int jj = e - NDOF * (int)ia;
+                    const long long k =
+                        (long long)NDOF * NDOF * ia + (MIRROR ? NDOF * jj + ii : NDOF * ii + jj);
+                    v[u] = HAS_B ? fma(scale, B[k], E[k]) : E[k];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (p[u] >= 0 && p[u] < wl) atomicAdd(&a[p[u]], v[u]);
+            }
+            for (int p = ii; p < wl; p += NDOF) out[cb + w0 + p] = a[p];
+        }
+    }
+}
+
+hipError_t csc_gather_inc_real(const CscBuild& b, long long N, const double* E, const double* B,
+                               double scale, bool mirror, double* out, hipStream_t s) {
+    if (b.nnz == 0 || N == 0) return hipSuccess;
+    if (b.ndof != 10) return hipErrorInvalidValue;
+    constexpr int kCols = 64 / 10;
+    const long long groups = (N + kIncWaves * kCols - 1) / (kIncWaves * kCols);
+    const unsigned g = (unsigned)(groups < 65536 * 4 ? groups : 65536 * 4);
+#define EM_INC_REAL(MIR, HB)                                                                    \
+    hipLaunchKernelGGL((csc_gather_inc_real_kernel<10, MIR, HB>), dim3(g), dim3(64 * kIncWaves), \
+                       0, s, b.indptr, b.inc_ptr, b.inc, b.pos, N, E, B, scale, out)
+    if (mirror) {
+        if (B)
+            EM_INC_REAL(true, true);
+        else
+            EM_INC_REAL(true, false);
+    } else {
+        if (B)
+            EM_INC_REAL(false, true);
+        else
+            EM_INC_REAL(false, false);
+    }
+#undef EM_INC_REAL
+    return hipGetLastError();
 }
 
 // Real-valued data (one double per COO entry): data_csc[q] = sum of its contributors.

@@ -88,6 +88,9 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
 hipError_t csc_build_ndof10(const int* t2f, long long n_tets, long long N, CscBuild& b,
                             long long& max_inc, hipStream_t s);
 // out[q] = sum of E[k] (+ scale * B[k]) over the contributors of entry q (real data).
+// Real-valued column-incidence gather (10-DOF maps; needs b.inc and b.pos).
+hipError_t csc_gather_inc_real(const CscBuild& b, long long N, const double* E, const double* B,
+                               double scale, bool mirror, double* out, hipStream_t s);
 hipError_t csc_gather_real(const long long* seg, const long long* perm, long long nnz,
                            const double* E, const double* B, double scale, double* out,
                            hipStream_t s);

@@ -158,8 +158,13 @@ class HeatCscMap:
         return out
 
     def to_csc(self, data: torch.Tensor, data_b: torch.Tensor | None = None, scale: float = 0.0,
-               out=None, stream=None) -> torch.Tensor:
-        """CSC values of ``data`` (+ ``scale * data_b``), float64 on the device."""
+               out=None, stream=None, segment_gather: bool = False,
+               block_symmetric: bool = False) -> torch.Tensor:
+        """CSC values of ``data`` (+ ``scale * data_b``), float64 on the device.
+        ``segment_gather`` selects the per-entry gather instead of the column-incidence one;
+        ``block_symmetric`` (every 10x10 block bit-symmetric: capacity blocks, conductance
+        blocks of a scalar or diagonal conductivity) reads each entry from its mirror. All
+        give the same result bit for bit."""
         if data.dtype != torch.float64 or data.numel() != self.nnz_coo:
             raise ValueError(f"data must be float64 with {self.nnz_coo} entries")
         data = data.contiguous()
@@ -169,8 +174,10 @@ class HeatCscMap:
             data_b = data_b.contiguous()
         if out is None:
             out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
+        flags = (_lib.EM_CSC_SEGMENT_GATHER if segment_gather else 0) | \
+            (_lib.EM_CSC_BLOCK_SYMMETRIC if block_symmetric else 0)
         check(_lib.lib().em_csc_to_csc_real(self._h, _vp(data), _vp(data_b), float(scale),
-                                            _vp(out), _stream_ptr(stream)))
+                                            _vp(out), flags, _stream_ptr(stream)))
         return out
 
 

@@ -343,10 +343,14 @@ int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream)
 /*
  * CSCMapping.to_csc for float64 data (scatter_to_csc_f8, csc_cast.py:180-194):
  * data_csc[q] = sum of data[k] (+ scale * data_b[k] when data_b is not NULL, e.g. a
- * transient K + M / dt) over the entry's COO contributors, in COO order.
+ * transient K + M / dt) over the entry's COO contributors, in COO order. flags: 0 (the
+ * column-incidence gather on heat maps), EM_CSC_BLOCK_SYMMETRIC (the same, reading each
+ * entry from its mirror: valid when every 10x10 block is bit-symmetric, as the capacity
+ * blocks and the conductance blocks of a scalar or diagonal conductivity are) or
+ * EM_CSC_SEGMENT_GATHER (the per-entry gather); all give the same result bit for bit.
  */
 int em_csc_to_csc_real(em_csc_map_t map, const double* data, const double* data_b, double scale,
-                       double* data_csc, void* stream);
+                       double* data_csc, int flags, void* stream);
 
 /*
  * HIP graphs of assembly launches. The reference re-runs its Numba builder per call

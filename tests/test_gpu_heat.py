@@ -126,3 +126,58 @@ def test_heat_csc_and_energy(wr90_field):
     assert (mapping.to_csc(Kd) - A).nnz == 0 or sp.linalg.norm(mapping.to_csc(Kd) - A) <= 1e-12
     Md, _ = tet_mass_matrix(f, np.ones(T))
     assert rel_fro(Md, M.cpu().numpy()) <= 1e-14
+
+
+def test_heat_csc_incidence_gather_equals_segment_gather(delaunay_field):
+    """The column-incidence gather of real data sums every CSC entry's contributors in COO
+    order, like the per-entry segment gather: the results are equal bit for bit, with and
+    without the fused K + s M combine, on random (not symmetric) block values."""
+    import torch
+    from emerge_amd.heat import HeatCscMap, HeatPlan
+    from emerge_amd.mesh import Legrange2Field
+    f = Legrange2Field(delaunay_field.mesh)
+    plan = HeatPlan.from_field(f)
+    cm = HeatCscMap(plan)
+    g = torch.Generator().manual_seed(11)
+    E = torch.randn(cm.nnz_coo, generator=g, dtype=torch.float64).cuda()
+    B = torch.randn(cm.nnz_coo, generator=g, dtype=torch.float64).cuda()
+    for args in ((E,), (E, B, 1e3)):
+        inc = cm.to_csc(*args)
+        seg = cm.to_csc(*args, segment_gather=True)
+        assert torch.equal(inc, seg)
+    # and against the sum in sorted-contributor order on the host
+    ex = cm.export("perm", "target_csc")
+    perm, tgt = ex["perm"].cpu().numpy(), ex["target_csc"].cpu().numpy()
+    q = tgt  # CSC entry of the k-th contributor in sorted order (csc_cast.py:162-177)
+    assert np.all(np.diff(q) >= 0)
+    segs = np.searchsorted(q, np.arange(cm.nnz + 1))
+    vals = E.cpu().numpy()[perm]
+    acc = np.zeros(len(segs) - 1)
+    lens = np.diff(segs)
+    for r in range(int(lens.max())):
+        m = lens > r
+        acc[m] += vals[segs[:-1][m] + r]
+    assert np.array_equal(acc, cm.to_csc(E).cpu().numpy())
+
+
+def test_heat_blocks_bit_symmetric_and_mirrored_gather(delaunay_field):
+    """With a diagonal conductivity every conductance block is bit-symmetric (the 10-term
+    form adds the off-diagonal coefficient pairs in one order for (i, j) and (j, i)), as is
+    every capacity block; the mirrored incidence gather then equals the segment gather bit
+    for bit."""
+    import torch
+    from emerge_amd.heat import HeatCscMap, HeatPlan
+    from emerge_amd.mesh import Legrange2Field
+    f = Legrange2Field(delaunay_field.mesh)
+    T = f.n_tets
+    rng = np.random.default_rng(8)
+    plan = HeatPlan.from_field(f)
+    K, M = plan.assemble(cond=torch.as_tensor(rng.uniform(1, 5, (T, 3))),
+                         rho_cp=torch.as_tensor(rng.uniform(1, 2, T)))
+    for X in (K, M):
+        b = X.reshape(T, 10, 10)
+        assert torch.equal(b, b.transpose(1, 2))
+    cm = HeatCscMap(plan)
+    ref = cm.to_csc(K, M, 250.0, segment_gather=True)
+    assert torch.equal(cm.to_csc(K, M, 250.0, block_symmetric=True), ref)
+    assert torch.equal(cm.to_csc(K, block_symmetric=True), cm.to_csc(K, segment_gather=True))

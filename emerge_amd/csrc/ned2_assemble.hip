@@ -296,6 +296,73 @@ __device__ __forceinline__ void store2(double2* blk, int e0, int e1, double v0r,
     __builtin_nontemporal_store(b, reinterpret_cast<f64x2*>(blk + e1));
 }
 
+// Streams a tile's blocks (the thread's two entries of every tet). KRE / MRE: the imaginary
+// parts of every K / M metric tensor of the tile are zero (a real mu_r / eps_r), so their
+// products (all +-0, which leave a +0 accumulator at +0) are skipped; the result is the
+// same bit for bit.
+template <typename R, int MODE, bool FULL, int TILE, bool KRE, bool MRE>
+__device__ __forceinline__ void stream_tile(const StreamRec<R, FULL, TILE>& rec, int cnt,
+                                            const int (&E)[2], const R (&tk)[2][6],
+                                            const R (&tm)[2][6], const R (&qk)[2][3],
+                                            const R (&qm)[2][3], const bool (&odd)[2],
+                                            const K0Pack& k0p, int nf, void* __restrict__ out0,
+                                            void* __restrict__ out1, long long freq_stride) {
+    using O = typename OutVec<R>::T;
+    for (int lt = 0; lt < cnt; ++lt) {
+        R gkr[6], gki[6], gmr[6], gmi[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            gkr[c] = rec.gk[lt][c][0];
+            gki[c] = KRE ? (R)0 : rec.gk[lt][c][1];
+            gmr[c] = rec.gm[lt][c][0];
+            gmi[c] = MRE ? (R)0 : rec.gm[lt][c][1];
+        }
+        const R s = rec.sgn[lt];
+        const long long base = (long long)rec.ia[lt] * 400;
+        R kr[2], ki[2], mr[2], mi[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            R a = 0, b = 0, c2 = 0, d = 0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                a += tk[q][c] * gkr[c];
+                if constexpr (!KRE) b += tk[q][c] * gki[c];
+                c2 += tm[q][c] * gmr[c];
+                if constexpr (!MRE) d += tm[q][c] * gmi[c];
+            }
+            if constexpr (FULL) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    a += qk[q][c] * rec.ak[lt][c][0];
+                    if constexpr (!KRE) b += qk[q][c] * rec.ak[lt][c][1];
+                    c2 += qm[q][c] * rec.am[lt][c][0];
+                    if constexpr (!MRE) d += qm[q][c] * rec.am[lt][c][1];
+                }
+            }
+            const R sg = odd[q] ? s : (R)1;
+            kr[q] = sg * a;
+            ki[q] = sg * b;
+            mr[q] = sg * c2;
+            mi[q] = sg * d;
+        }
+        if (MODE == MODE_A) {
+            // G_M already carries -k0^2
+            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0] + mr[0], ki[0] + mi[0],
+                   kr[1] + mr[1], ki[1] + mi[1]);
+        } else if (MODE == MODE_KM) {
+            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0], ki[0], kr[1], ki[1]);
+            store2(reinterpret_cast<O*>(out1) + base, E[0], E[1], mr[0], mi[0], mr[1], mi[1]);
+        } else {
+            for (int f = 0; f < nf; ++f) {
+                const R kf = (R)k0p.v[f];
+                store2(reinterpret_cast<O*>(out0) + (long long)f * freq_stride + base, E[0], E[1],
+                       kr[0] - kf * mr[0], ki[0] - kf * mi[0], kr[1] - kf * mr[1],
+                       ki[1] - kf * mi[1]);
+            }
+        }
+    }
+}
+
 // One workgroup per tile: up to TILE tets that share one vertex-rank code (so every
 // output slot maps to the same canonical pair for all of them). Threads 0..199 own two
 // output entries of every tet's 20x20 block, with the reference-tensor coefficients of
@@ -307,13 +374,13 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
     MeshDev m, MatDesc eps, MatDesc mu, K0Pack k0p, int nf, const int* __restrict__ order,
     const int2* __restrict__ tiles, void* __restrict__ out0, void* __restrict__ out1,
     long long freq_stride) {
-    using O = typename OutVec<R>::T;
     __shared__ StreamRec<R, FULL, TILE> rec;
     const int2 td = tiles[blockIdx.x];
     const int start = td.x, cnt = td.y & 0xffff, code = td.y >> 16;
     if (cnt == 0) return;  // padding tile (uniform across the workgroup)
     const double mscale = (MODE == MODE_A) ? -k0p.v[0] : 1.0;
 
+    int kre = 1, mre = 1;  // this thread's tets have real K / M metric tensors
     for (int lt = threadIdx.x; lt < cnt; lt += kBlock) {
         const int ia = order[start + lt];
         const long long t = m.tetids ? (long long)m.tetids[ia] : (long long)ia;
@@ -321,6 +388,18 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
         tet_geometry<R, FULL>(m, eps, mu, t, mscale, rec.gk[lt], rec.gm[lt],
                               rec.ak[FULL ? lt : 0], rec.am[FULL ? lt : 0], c, rec.sgn[lt]);
         rec.ia[lt] = ia;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            kre &= rec.gk[lt][k][1] == (R)0;
+            mre &= rec.gm[lt][k][1] == (R)0;
+        }
+        if constexpr (FULL) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                kre &= rec.ak[lt][k][1] == (R)0;
+                mre &= rec.am[lt][k][1] == (R)0;
+            }
+        }
     }
 
     // the two output slots of this thread and their coefficients: complex64 entries go
@@ -353,62 +432,25 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
         }
         odd[q] = (ii >= 6) != (jj >= 6);
     }
-    __syncthreads();
+    // the barrier before streaming, with the tile's reality flags (uniform afterwards)
+    const bool kreal = __syncthreads_and(kre) != 0;
+    const bool mreal = __syncthreads_and(mre) != 0;
     if (!active) return;
-
-    for (int lt = 0; lt < cnt; ++lt) {
-        R gkr[6], gki[6], gmr[6], gmi[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            gkr[c] = rec.gk[lt][c][0];
-            gki[c] = rec.gk[lt][c][1];
-            gmr[c] = rec.gm[lt][c][0];
-            gmi[c] = rec.gm[lt][c][1];
-        }
-        const R s = rec.sgn[lt];
-        const long long base = (long long)rec.ia[lt] * 400;
-        R kr[2], ki[2], mr[2], mi[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            R a = 0, b = 0, c2 = 0, d = 0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                a += tk[q][c] * gkr[c];
-                b += tk[q][c] * gki[c];
-                c2 += tm[q][c] * gmr[c];
-                d += tm[q][c] * gmi[c];
-            }
-            if constexpr (FULL) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    a += qk[q][c] * rec.ak[lt][c][0];
-                    b += qk[q][c] * rec.ak[lt][c][1];
-                    c2 += qm[q][c] * rec.am[lt][c][0];
-                    d += qm[q][c] * rec.am[lt][c][1];
-                }
-            }
-            const R sg = odd[q] ? s : (R)1;
-            kr[q] = sg * a;
-            ki[q] = sg * b;
-            mr[q] = sg * c2;
-            mi[q] = sg * d;
-        }
-        if (MODE == MODE_A) {
-            // G_M already carries -k0^2
-            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0] + mr[0], ki[0] + mi[0],
-                   kr[1] + mr[1], ki[1] + mi[1]);
-        } else if (MODE == MODE_KM) {
-            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0], ki[0], kr[1], ki[1]);
-            store2(reinterpret_cast<O*>(out1) + base, E[0], E[1], mr[0], mi[0], mr[1], mi[1]);
-        } else {
-            for (int f = 0; f < nf; ++f) {
-                const R kf = (R)k0p.v[f];
-                store2(reinterpret_cast<O*>(out0) + (long long)f * freq_stride + base, E[0], E[1],
-                       kr[0] - kf * mr[0], ki[0] - kf * mi[0], kr[1] - kf * mr[1],
-                       ki[1] - kf * mi[1]);
-            }
-        }
+#define EM_STREAM(KR, MR)                                                                     \
+    stream_tile<R, MODE, FULL, TILE, KR, MR>(rec, cnt, E, tk, tm, qk, qm, odd, k0p, nf, out0, \
+                                            out1, freq_stride)
+    if (kreal) {
+        if (mreal)
+            EM_STREAM(true, true);
+        else
+            EM_STREAM(true, false);
+    } else {
+        if (mreal)
+            EM_STREAM(false, true);
+        else
+            EM_STREAM(false, false);
     }
+#undef EM_STREAM
 }
 
 // Launch over all tiles; MODE_AF launches once per chunk of up to kMaxK0 wavenumbers

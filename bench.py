@@ -197,10 +197,20 @@ def suite_backward(args) -> dict:
     fn = lambda: plan.backward(g, k0, _lib.EM_MAT_SCALAR, ge, _lib.EM_MAT_DIAG, gm)  # noqa: E731
     ms, _ = time_launches(fn, 10, 2)
     log(f"backward: {T} tets, {ms:.3f} ms")
+    # full (T, 3, 3) leaves (antisymmetric parts included)
+    gef = torch.zeros((T, 9), dtype=torch.complex128, device="cuda")
+    gmf = torch.zeros((T, 9), dtype=torch.complex128, device="cuda")
+    fnf = lambda: plan.backward(g, k0, _lib.EM_MAT_FULL, gef, _lib.EM_MAT_FULL, gmf)  # noqa: E731
+    ms_full, _ = time_launches(fnf, 10, 2)
+    log(f"backward full tensors: {ms_full:.3f} ms")
+    bf = 400 * 16 + 16 + 24 * f.mesh.n_nodes / T + 2 * 144
     b = 400 * 16 + 16 + 24 * f.mesh.n_nodes / T + 16 + 48
     return {"n_tets": T, "ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
             "gb_per_s": T * b / (ms * 1e-3) / 1e9, "bytes_per_tet": b,
-            "leaves": "eps_r (T,) complex, mu_r^-1 (T,3) complex"}
+            "leaves": "eps_r (T,) complex, mu_r^-1 (T,3) complex",
+            "full_tensor_leaves": {"ms_per_launch": ms_full, "tets_per_s": T / (ms_full * 1e-3),
+                                   "gb_per_s": T * bf / (ms_full * 1e-3) / 1e9,
+                                   "bytes_per_tet": bf}}
 
 
 def suite_csc_pec(args) -> dict:

@@ -10,11 +10,9 @@
 //         - k0^2 (sum_c gm_c TSYM_M[p][c] + sa sum_c am_c TANTI_M[p][c]))
 // for its canonical pair p, so with w = sg * (g(ii,jj) + g(jj,ii)) (g(ii,ii) once)
 //   dL/dgk_c = sum_p w TSYM_K[p][c],  dL/dgm_c = -k0^2 sum_p w TSYM_M[p][c]
-// (and the antisymmetric parts with sa). Per round of 8 tets the 210 pair-threads write
-// the signed weights w into LDS (reading the upstream block straight from HBM, each
-// entry once), then 192 threads form the 8 x 12 dot products of length 210 with the
-// reference tensors held in LDS (a small GEMM). Then, from G_M = C eps C^T / |det J| and
-// G_K = Jc mu^-1 Jc^T / |det J| (Jc rows = columns of J):
+// (and the antisymmetric parts with sa): 12 (18) dot products of length 210 per tet, a
+// small GEMM against the reference tensors (ned2_backward_wave_kernel below). Then, from
+// G_M = C eps C^T / |det J| and G_K = Jc mu^-1 Jc^T / |det J| (Jc rows = columns of J):
 //   dL/deps = C^T W_M C / |det J|,   dL/dmu^-1 = Jc^T W_K Jc / |det J|
 // reduced to the leaf's kind (scalar: trace, diagonal: diagonal, full: all 9).
 #include <hip/hip_runtime.h>
@@ -28,32 +26,6 @@ namespace em {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kTile = 64;
-
-// Work split of one round of NW tets: the dot-product phase runs NW x NC x NSPLIT threads.
-template <bool FULL>
-struct Shape {
-    static constexpr int NW = 8;                  // tets per round
-    static constexpr int NC = FULL ? 18 : 12;     // complex outputs per tet (K then M)
-    static constexpr int NSPLIT = FULL ? 1 : 2;   // pair ranges per output
-};
-
-template <bool FULL>
-struct BwdSmem {
-    static constexpr int NW = Shape<FULL>::NW, NC = Shape<FULL>::NC, NS = Shape<FULL>::NSPLIT;
-    double Jc[kTile][9];
-    double C[kTile][9];
-    double iad[kTile];
-    double sgn[kTile];
-    long long tet[kTile];
-    int code[kTile];
-    double T[NED2_NPAIR][NC];          // reference tensors, M part scaled by -k0^2
-    double2 W[NW][NED2_NPAIR];         // signed pair weights
-    double2 Wa[FULL ? NW : 1][NED2_NPAIR];  // times the antisymmetric sign
-    double2 R[NS][NW][NC];
-    unsigned char slot[64][20];
-};
 
 // g_out (tet-major, complex) for a 3x3 complex gradient Gr (row-major) reduced to kind.
 __device__ __forceinline__ void write_leaf(double2* out, long long t, int kind,
@@ -115,180 +87,41 @@ __device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool
     }
 }
 
-template <bool FULL>
-__global__ __launch_bounds__(kBlock) void ned2_backward_kernel(
-    MeshDev m, const double2* __restrict__ grad, double k0sq, int eps_kind, int mu_kind,
-    double2* __restrict__ g_eps, double2* __restrict__ g_mu) {
-    using S = Shape<FULL>;
-    constexpr int NW = S::NW, NC = S::NC, NS = S::NSPLIT;
-    constexpr int NP = NC;  // doubles per matrix in the flattened (re, im) result
-    __shared__ BwdSmem<FULL> sm;
-    for (int i = threadIdx.x; i < 64 * 20; i += kBlock) sm.slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
-    for (int i = threadIdx.x; i < NED2_NPAIR * NC; i += kBlock) {
-        const int p = i / NC, c = i % NC;
-        double v;
-        if (FULL) {
-            v = c < 6 ? TSYM_K[p][c] : c < 9 ? TANTI_K[p][c - 6]
-                : c < 15 ? -k0sq * TSYM_M[p][c - 9] : -k0sq * TANTI_M[p][c - 15];
-        } else {
-            v = c < 6 ? TSYM_K[p][c] : -k0sq * TSYM_M[p][c - 6];
-        }
-        sm.T[p][c] = v;
-    }
-
-    const long long tile0 = (long long)blockIdx.x * kTile;
-    const int ntile = (int)min((long long)kTile, m.n_assy - tile0);
-    if (threadIdx.x < ntile) {
-        const int lt = threadIdx.x;
-        const long long ia = tile0 + lt;
-        const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
-        const int4 nd = m.tets[t];
-        const int n[4] = {nd.x, nd.y, nd.z, nd.w};
-        int r[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-            r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
-        const int s0 = min(n[0], n[1]), s1 = max(n[0], n[1]);
-        const int s2 = min(n[2], n[3]), s3 = max(n[2], n[3]);
-        const int w0 = min(s0, s2), t2 = max(s0, s2);
-        const int t1 = min(s1, s3), w3 = max(s1, s3);
-        const int w1 = min(t1, t2), w2 = max(t1, t2);
-        const double4 X0 = m.xyz[w0], X1 = m.xyz[w1], X2 = m.xyz[w2], X3 = m.xyz[w3];
-        const double J[9] = {X1.x - X0.x, X1.y - X0.y, X1.z - X0.z, X2.x - X0.x, X2.y - X0.y,
-                             X2.z - X0.z, X3.x - X0.x, X3.y - X0.y, X3.z - X0.z};
-        const double C[9] = {J[4] * J[8] - J[5] * J[7], J[5] * J[6] - J[3] * J[8],
-                             J[3] * J[7] - J[4] * J[6], J[7] * J[2] - J[8] * J[1],
-                             J[8] * J[0] - J[6] * J[2], J[6] * J[1] - J[7] * J[0],
-                             J[1] * J[5] - J[2] * J[4], J[2] * J[3] - J[0] * J[5],
-                             J[0] * J[4] - J[1] * J[3]};
-        const double det = J[0] * C[0] + J[1] * C[1] + J[2] * C[2];
-        const int inv = (r[0] > r[1]) + (r[0] > r[2]) + (r[0] > r[3]) + (r[1] > r[2]) +
-                        (r[1] > r[3]) + (r[2] > r[3]);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            sm.Jc[lt][i] = J[i];
-            sm.C[lt][i] = C[i];
-        }
-        sm.iad[lt] = 1.0 / fabs(det);
-        sm.sgn[lt] = ((det < 0.0) != (inv & 1)) ? -1.0 : 1.0;
-        sm.code[lt] = r[0] * 16 + r[1] * 4 + r[2];
-        sm.tet[lt] = t;
-    }
-    const int j = threadIdx.x;
-    const bool pair_thread = j < NED2_NPAIR;
-    const int pj = pair_thread ? j : 0;
-    const int a = PAIR_A[pj], b = PAIR_B[pj];
-    const bool aodd = a >= 6, bodd = b >= 6;
-    // dot-phase role: (u, c, split)
-    const bool dot_thread = threadIdx.x < NW * NC * NS;
-    const int du = threadIdx.x / (NC * NS);
-    const int dc = (threadIdx.x / NS) % NC;
-    const int dsp = threadIdx.x % NS;
-    const bool anti_c = FULL && ((dc >= 6 && dc < 9) || dc >= 15);
-    const int p_lo = dsp * ((NED2_NPAIR + NS - 1) / NS);
-    const int p_hi = min(NED2_NPAIR, p_lo + (NED2_NPAIR + NS - 1) / NS);
-    __syncthreads();
-
-    for (int lt0 = 0; lt0 < ntile; lt0 += NW) {
-        const int nvalid = min(NW, ntile - lt0);
-        // pair phase: signed weight of every canonical pair of every tet of the round
-        if (pair_thread) {
-            for (int u = 0; u < nvalid; ++u) {
-                const int lt = lt0 + u;
-                const int code = sm.code[lt];
-                const int ra = sm.slot[code][a], rb = sm.slot[code][b];
-                const double2* blk = grad + (tile0 + lt) * 400;
-                double2 g = blk[ra * 20 + rb];
-                if (ra != rb) {
-                    const double2 g2 = blk[rb * 20 + ra];
-                    g.x += g2.x;
-                    g.y += g2.y;
-                }
-                const double s = sm.sgn[lt];
-                const double sg = (aodd ? s : 1.0) * (bodd ? s : 1.0);
-                sm.W[u][j] = make_double2(sg * g.x, sg * g.y);
-                if constexpr (FULL) {
-                    const double sa = (ra < rb) ? -sg : sg;
-                    sm.Wa[u][j] = make_double2(sa * g.x, sa * g.y);
-                }
-            }
-        }
-        __syncthreads();
-        // dot phase: R[split][u][c] = sum over pairs of W[u][p] * T[p][c]
-        if (dot_thread && du < nvalid) {
-            const double2* w = (FULL && anti_c) ? sm.Wa[FULL ? du : 0] : sm.W[du];
-            double re = 0.0, im = 0.0;
-            for (int p = p_lo; p < p_hi; ++p) {
-                const double2 v = w[p];
-                const double tv = sm.T[p][dc];
-                re += v.x * tv;
-                im += v.y * tv;
-            }
-            sm.R[dsp][du][dc] = make_double2(re, im);
-        }
-        __syncthreads();
-        if (threadIdx.x < nvalid) {
-            const int u = threadIdx.x;
-            const int lt = lt0 + u;
-            double w[2 * NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                double re = 0.0, im = 0.0;
-#pragma unroll
-                for (int q = 0; q < NS; ++q) {
-                    re += sm.R[q][u][c].x;
-                    im += sm.R[q][u][c].y;
-                }
-                w[2 * c] = re;
-                w[2 * c + 1] = im;
-            }
-            double Wre[9], Wim[9], Ore[9], Oim[9], A[9];
-            const long long t = sm.tet[lt];
-            const double iad = sm.iad[lt];
-            if (g_mu) {
-                unsplit(w, w + 12, FULL, Wre, Wim);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) A[i] = sm.Jc[lt][i];
-                pullback(A, Wre, Wim, iad, Ore, Oim);
-                write_leaf(g_mu, t, mu_kind, Ore, Oim);
-            }
-            if (g_eps) {
-                unsplit(w + NP, w + NP + 12, FULL, Wre, Wim);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) A[i] = sm.C[lt][i];
-                pullback(A, Wre, Wim, iad, Ore, Oim);
-                write_leaf(g_eps, t, eps_kind, Ore, Oim);
-            }
-        }
-        __syncthreads();  // the next round's dot phase rewrites R
-    }
-}
-
-// Scalar/diagonal leaves (no antisymmetric parts): one wavefront per 64-tet tile, no
-// barriers across wavefronts. The wavefront's four 16-lane rows own three of the 12
-// components each (row r: components 3r .. 3r+2) and lane j of a row owns pairs
-// 14j .. 14j+13 (lane 15 none), with those 3 x 14 reference-tensor coefficients in
-// registers (the M ones times -k0^2); so one LDS read of a pair weight feeds six FMAs and
-// the 16 partial sums of a row are reduced by DPP lane moves (no LDS traffic). The
-// upstream 20x20 blocks are read coalesced into registers kWAhead tets ahead and staged
-// in LDS a group at a time (the loads of the next group are in flight while the current
-// one is processed). Per tet every lane forms the signed weights of pairs lane,
-// lane + 64, ... into LDS; the 12 sums of 16 tets are kept in LDS and 16 lanes then apply
-// the chain rule, one tet each.
+// Wavefront kernel: one wavefront per 64-tet tile, no barriers across wavefronts. The
+// wavefront's four 16-lane rows own three of the 12 symmetric components each (row r:
+// components 3r .. 3r+2; K then M) and lane j of a row owns pairs 14j .. 14j+13 (lane 15
+// none), with those 3 x 14 reference-tensor coefficients in registers (the M ones times
+// -k0^2); so one LDS read of a pair weight feeds six FMAs and the 16 partial sums of a row
+// are reduced by DPP lane moves (no LDS traffic). With full tensor leaves (FULL) a second
+// pass over the antisymmetric weights gives the 3 + 3 antisymmetric components (rows 0
+// and 1). The upstream 20x20 blocks are read coalesced into registers one group ahead and
+// staged in LDS a group at a time (the loads of the next group are in flight while the
+// current one is processed). Per tet every lane forms the signed weights of pairs lane,
+// lane + 64, ... into LDS; the sums of 16 tets are kept in LDS and the lanes holding those
+// tets' geometry then apply the chain rule.
 constexpr int kWTile = 64;     // tets per wavefront tile (one lane per tet for geometry)
 constexpr int kWChunk = 16;    // tets per chain-rule batch
 constexpr int kWWaves = 4;     // wavefronts per workgroup
-constexpr int kWAhead = 1;     // upstream blocks staged (and prefetched) per group
 constexpr int kWComps = 3;     // components per lane (per 16-lane row)
 constexpr int kWPairs = 14;    // pairs per lane (15 x 14 = 210)
 constexpr int kWStride = 15;   // LDS slots per lane range (one pad: no bank conflicts)
 constexpr int kWSlots = 16 * kWStride;
 
-// 13.3 KB per wavefront: two 4-wavefront workgroups fit a CU, two wavefronts per SIMD.
+template <bool FULL>
+struct WaveCfg {
+    static constexpr int AHEAD = 1;               // upstream blocks staged per group
+    static constexpr int NRES = FULL ? 18 : 12;   // K sym, M sym (, K anti, M anti)
+};
+
+// Scalar/diagonal: 13.3 KB per wavefront (two 4-wavefront workgroups per CU, two
+// wavefronts per SIMD). Full: 18.7 KB per wavefront plus the antisymmetric coefficient
+// table, one wavefront per SIMD (450 VGPRs).
+template <bool FULL>
 struct WaveSmem {
-    double2 blk[kWAhead][400];
-    double2 w[kWAhead][kWSlots];  // pair p at (p / 14) * 15 + p % 14; lane 15's range is zero
-    double2 res[kWChunk][12];
+    double2 blk[WaveCfg<FULL>::AHEAD][400];
+    double2 w[WaveCfg<FULL>::AHEAD][kWSlots];  // pair p at (p / 14) * 15 + p % 14
+    double2 wa[FULL ? WaveCfg<FULL>::AHEAD : 1][FULL ? kWSlots : 1];  // antisymmetric
+    double2 res[kWChunk][WaveCfg<FULL>::NRES];
 };
 
 // Cofactor matrix of J (row-major; C[3p + r]).
@@ -397,20 +230,23 @@ struct LaneGeom {
     int code, neg;
 };
 
-// One group of nq <= kWAhead tets of a wavefront tile (lt0 ..), their upstream blocks
-// staged in sm.blk: the signed pair weights and the 12 sums of every tet of the group.
-// When a 16-tet chunk is complete its chain rule runs on the lanes that hold the chunk's
-// geometry.
-__device__ __forceinline__ void wave_group(WaveSmem& sm, const unsigned char (*slot)[20],
+// One group of nq <= AHEAD tets of a wavefront tile (lt0 ..), their upstream blocks
+// staged in sm.blk: the signed pair weights and the component sums of every tet of the
+// group. When a 16-tet chunk is complete its chain rule runs on the lanes that hold the
+// chunk's geometry.
+template <bool FULL>
+__device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned char (*slot)[20],
                                            const double (&coef)[kWComps][kWPairs],
+                                           const double4* coefa,
                                            const int (&pa)[4], const int (&pb)[4],
                                            const int (&pw)[4], const LaneGeom& gm, int lane,
                                            int row, int j, int lt0, int nq, int ntile,
                                            int eps_kind, int mu_kind,
                                            double2* __restrict__ g_eps,
                                            double2* __restrict__ g_mu) {
+    constexpr int AH = WaveCfg<FULL>::AHEAD;
 #pragma unroll
-    for (int q = 0; q < kWAhead; ++q) {
+    for (int q = 0; q < AH; ++q) {
         const int lt = min(lt0 + q, ntile - 1);
         const int code = __builtin_amdgcn_readlane(gm.code, lt);
         const double s = __builtin_amdgcn_readlane(gm.neg, lt) ? -1.0 : 1.0;
@@ -429,36 +265,67 @@ __device__ __forceinline__ void wave_group(WaveSmem& sm, const unsigned char (*s
                 }
                 const double sgp = (a >= 6 ? s : 1.0) * (b >= 6 ? s : 1.0);
                 sm.w[q][pw[k]] = make_double2(sgp * v.x, sgp * v.y);
+                if constexpr (FULL) {
+                    // antisymmetric sign: negative when the pair's lower canonical function
+                    // sits in the lower reference slot
+                    const double sa = ra < rb ? -sgp : sgp;
+                    sm.wa[q][pw[k]] = make_double2(sa * v.x, sa * v.y);
+                }
             }
         }
     }
     wave_sync();
-    double acc[kWAhead][kWComps][2];
+    double acc[AH][kWComps][2], acca[AH][kWComps][2];
 #pragma unroll
-    for (int q = 0; q < kWAhead; ++q)
+    for (int q = 0; q < AH; ++q)
 #pragma unroll
-        for (int c = 0; c < kWComps; ++c) acc[q][c][0] = acc[q][c][1] = 0.0;
+        for (int c = 0; c < kWComps; ++c) {
+            acc[q][c][0] = acc[q][c][1] = 0.0;
+            acca[q][c][0] = acca[q][c][1] = 0.0;
+        }
 #pragma unroll
     for (int k = 0; k < kWPairs; ++k) {
 #pragma unroll
-        for (int q = 0; q < kWAhead; ++q) {
+        for (int q = 0; q < AH; ++q) {
             const double2 v = sm.w[q][j * kWStride + k];
 #pragma unroll
             for (int c = 0; c < kWComps; ++c) {
                 acc[q][c][0] += v.x * coef[c][k];
                 acc[q][c][1] += v.y * coef[c][k];
             }
+            if constexpr (FULL) {
+                // antisymmetric pass: rows 0 and 1 (K, M); rows 2 and 3 repeat them and
+                // their sums are not used. Lane 15's weights are zero.
+                const double2 va = sm.wa[q][j * kWStride + k];
+                const double4 ca =
+                    coefa[(row & 1) * NED2_NPAIR + min(kWPairs * j + k, NED2_NPAIR - 1)];
+                const double cc[3] = {ca.x, ca.y, ca.z};
+#pragma unroll
+                for (int c = 0; c < kWComps; ++c) {
+                    acca[q][c][0] += va.x * cc[c];
+                    acca[q][c][1] += va.y * cc[c];
+                }
+            }
         }
     }
 #pragma unroll
-    for (int q = 0; q < kWAhead; ++q) {
-        double2 mine = make_double2(0.0, 0.0);
+    for (int q = 0; q < AH; ++q) {
+        double2 mine = make_double2(0.0, 0.0), minea = make_double2(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < kWComps; ++c) {
             const double re = row_sum(acc[q][c][0]), im = row_sum(acc[q][c][1]);
             if (j == c) mine = make_double2(re, im);
+            if constexpr (FULL) {
+                const double ra = row_sum(acca[q][c][0]), ia = row_sum(acca[q][c][1]);
+                if (j == c) minea = make_double2(ra, ia);
+            }
         }
-        if (q < nq && j < kWComps) sm.res[(lt0 + q) % kWChunk][kWComps * row + j] = mine;
+        if (q < nq && j < kWComps) {
+            double2* r = sm.res[(lt0 + q) % kWChunk];
+            r[kWComps * row + j] = mine;
+            // antisymmetric K (row 0) and M (row 1) after the 12 symmetric components
+            if (FULL && row < 2) r[12 + kWComps * row + j] = minea;
+        }
     }
     wave_sync();
     const int last = lt0 + nq - 1;
@@ -468,34 +335,83 @@ __device__ __forceinline__ void wave_group(WaveSmem& sm, const unsigned char (*s
     const int c0 = last - u;
     if (lane >= c0 && lane <= last) {
         const double2* r = sm.res[lane - c0];
-        if (g_mu) write_diag_leaf(g_mu, gm.tet, mu_kind, gm.Jc, r, gm.iad);
-        if (g_eps) {
-            double C[9];
-            cofactors(gm.Jc, C);
-            write_diag_leaf(g_eps, gm.tet, eps_kind, C, r + 6, gm.iad);
+        if constexpr (!FULL) {
+            if (g_mu) write_diag_leaf(g_mu, gm.tet, mu_kind, gm.Jc, r, gm.iad);
+            if (g_eps) {
+                double C[9];
+                cofactors(gm.Jc, C);
+                write_diag_leaf(g_eps, gm.tet, eps_kind, C, r + 6, gm.iad);
+            }
+        } else {
+            double ws[12], wa[6], Wre[9], Wim[9], Ore[9], Oim[9];
+            if (g_mu) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    ws[2 * c] = r[c].x;
+                    ws[2 * c + 1] = r[c].y;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    wa[2 * c] = r[12 + c].x;
+                    wa[2 * c + 1] = r[12 + c].y;
+                }
+                unsplit(ws, wa, true, Wre, Wim);
+                pullback(gm.Jc, Wre, Wim, gm.iad, Ore, Oim);
+                write_leaf(g_mu, gm.tet, mu_kind, Ore, Oim);
+            }
+            if (g_eps) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    ws[2 * c] = r[6 + c].x;
+                    ws[2 * c + 1] = r[6 + c].y;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    wa[2 * c] = r[15 + c].x;
+                    wa[2 * c + 1] = r[15 + c].y;
+                }
+                double C[9];
+                cofactors(gm.Jc, C);
+                unsplit(ws, wa, true, Wre, Wim);
+                pullback(C, Wre, Wim, gm.iad, Ore, Oim);
+                write_leaf(g_eps, gm.tet, eps_kind, Ore, Oim);
+            }
         }
     }
     wave_sync();
 }
 
-__global__ __launch_bounds__(64 * kWWaves) __attribute__((amdgpu_waves_per_eu(2, 2)))
+template <bool FULL>
+__global__ __launch_bounds__(64 * kWWaves)
+__attribute__((amdgpu_waves_per_eu(FULL ? 1 : 2, FULL ? 8 : 2)))
 void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, double k0sq,
                                int eps_kind, int mu_kind, double2* __restrict__ g_eps,
                                double2* __restrict__ g_mu) {
-    __shared__ WaveSmem smem[kWWaves];
+    constexpr int AH = WaveCfg<FULL>::AHEAD;
+    __shared__ WaveSmem<FULL> smem[kWWaves];
     __shared__ unsigned char slot[64][20];
+    // antisymmetric coefficients (K; M times -k0^2), one padded row per pair
+    __shared__ double4 coefa[FULL ? 2 : 1][FULL ? NED2_NPAIR : 1];
     for (int i = threadIdx.x; i < 64 * 20; i += 64 * kWWaves)
         slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
+    if constexpr (FULL) {
+        for (int i = threadIdx.x; i < 2 * NED2_NPAIR; i += 64 * kWWaves) {
+            const int r = i / NED2_NPAIR, p = i % NED2_NPAIR;
+            coefa[r][p] = r == 0 ? make_double4(TANTI_K[p][0], TANTI_K[p][1], TANTI_K[p][2], 0.0)
+                                 : make_double4(-k0sq * TANTI_M[p][0], -k0sq * TANTI_M[p][1],
+                                                -k0sq * TANTI_M[p][2], 0.0);
+        }
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    WaveSmem& sm = smem[wave];
+    WaveSmem<FULL>& sm = smem[wave];
     const long long tile0 = ((long long)blockIdx.x * kWWaves + wave) * kWTile;
     if (tile0 >= m.n_assy) return;  // whole wavefront
     const int ntile = (int)min((long long)kWTile, m.n_assy - tile0);
-    // the upstream blocks of the next kWAhead tets, in flight while the current ones run
-    double bre[kWAhead][7], bim[kWAhead][7];
+    // the upstream blocks of the next AH tets, in flight while the current ones run
+    double bre[AH][7], bim[AH][7];
 #pragma unroll
-    for (int q = 0; q < kWAhead; ++q)
+    for (int q = 0; q < AH; ++q)
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             const int e = lane + 64 * k;
@@ -527,8 +443,11 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
     }
     // lane 15's range (read with zero coefficients) is never written by the pair phase
 #pragma unroll
-    for (int q = 0; q < kWAhead; ++q)
-        if (lane < kWStride) sm.w[q][15 * kWStride + lane] = make_double2(0.0, 0.0);
+    for (int q = 0; q < AH; ++q)
+        if (lane < kWStride) {
+            sm.w[q][15 * kWStride + lane] = make_double2(0.0, 0.0);
+            if (FULL) sm.wa[FULL ? q : 0][FULL ? 15 * kWStride + lane : 0] = make_double2(0.0, 0.0);
+        }
     // geometry of the lane's tet
     LaneGeom gm;
     {
@@ -536,29 +455,29 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
         bwd_geometry(m, tile0 + min(lane, ntile - 1), gm.Jc, C, gm.iad, gm.tet, gm.code, sg);
         gm.neg = sg < 0.0;
     }
-    for (int lt0 = 0; lt0 < ntile; lt0 += kWAhead) {
+    for (int lt0 = 0; lt0 < ntile; lt0 += AH) {
         // stage the blocks, then start the loads of the next group into the same registers
 #pragma unroll
-        for (int q = 0; q < kWAhead; ++q)
+        for (int q = 0; q < AH; ++q)
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
                 const int e = lane + 64 * k;
                 if (e < 400) sm.blk[q][e] = make_double2(bre[q][k], bim[q][k]);
             }
 #pragma unroll
-        for (int q = 0; q < kWAhead; ++q)
+        for (int q = 0; q < AH; ++q)
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
                 const int e = lane + 64 * k;
-                if (e < 400 && lt0 + kWAhead + q < ntile) {
-                    const double2 v = grad[(tile0 + lt0 + kWAhead + q) * 400 + e];
+                if (e < 400 && lt0 + AH + q < ntile) {
+                    const double2 v = grad[(tile0 + lt0 + AH + q) * 400 + e];
                     bre[q][k] = v.x;
                     bim[q][k] = v.y;
                 }
             }
         wave_sync();
-        wave_group(sm, slot, coef, pa, pb, pw, gm, lane, row, j, lt0,
-                   min(kWAhead, ntile - lt0), ntile, eps_kind, mu_kind, g_eps, g_mu);
+        wave_group<FULL>(sm, slot, coef, &coefa[0][0], pa, pb, pw, gm, lane, row, j, lt0,
+                         min(AH, ntile - lt0), ntile, eps_kind, mu_kind, g_eps, g_mu);
     }
 }
 
@@ -566,17 +485,16 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
 
 hipError_t launch_backward(const MeshDev& m, const double2* grad, double k0sq, int eps_kind,
                            int mu_kind, double2* g_eps, double2* g_mu, hipStream_t stream) {
-    const long long ntiles = (m.n_assy + kTile - 1) / kTile;
-    if (ntiles == 0) return hipSuccess;
+    if (m.n_assy <= 0) return hipSuccess;
     const bool full = (g_eps && eps_kind == EM_MAT_FULL) || (g_mu && mu_kind == EM_MAT_FULL);
+    const long long wtiles = (m.n_assy + kWTile - 1) / kWTile;
+    const unsigned blocks = (unsigned)((wtiles + kWWaves - 1) / kWWaves);
     if (full) {
-        hipLaunchKernelGGL((ned2_backward_kernel<true>), dim3((unsigned)ntiles), dim3(kBlock), 0,
+        hipLaunchKernelGGL(ned2_backward_wave_kernel<true>, dim3(blocks), dim3(64 * kWWaves), 0,
                            stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
     } else {
-        const long long wtiles = (m.n_assy + kWTile - 1) / kWTile;
-        const unsigned blocks = (unsigned)((wtiles + kWWaves - 1) / kWWaves);
-        hipLaunchKernelGGL(ned2_backward_wave_kernel, dim3(blocks), dim3(64 * kWWaves), 0, stream,
-                           m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+        hipLaunchKernelGGL(ned2_backward_wave_kernel<false>, dim3(blocks), dim3(64 * kWWaves), 0,
+                           stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
     }
     return hipGetLastError();
 }

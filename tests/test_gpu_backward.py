@@ -192,3 +192,34 @@ def test_subset_plan_scalar_and_diag_leaves(setup):
         ref[ids] = (dK * G).reshape(-1, 400).sum(dim=1)
         assert torch.linalg.norm(m.grad[:, d] - ref) <= 1e-12 * torch.linalg.norm(ref), d
         assert torch.all(m.grad[removed, d] == 0)
+
+
+def test_full_and_diagonal_leaves_together(setup):
+    """A full (T,3,3) eps_r leaf with a diagonal mu_r^-1 leaf (the full-tensor kernel writes
+    each leaf in its own kind), against the linear-term reference."""
+    import torch
+    from emerge_amd.autograd import assemble_a, bilinear_form
+    s = setup
+    plan, k0, T = s["plan"], s["k0"], s["T"]
+    rng = np.random.default_rng(9)
+    E = torch.tensor(rng.normal(size=(T, 3, 3)) * 0.2 + 3 * np.eye(3) - 0.05j, device="cuda")
+    e = E.clone().requires_grad_(True)
+    m = s["muinv"].clone().requires_grad_(True)
+    L = bilinear_form(assemble_a(plan, e, m, k0), s["rows"], s["cols"], s["lam"], s["x"])
+    L.backward()
+    G = s["lam"][s["rows"]] * torch.conj(s["x"][s["cols"]])
+    zero9 = torch.zeros((T, 3, 3), dtype=torch.complex128, device="cuda")
+    zero3 = torch.zeros((T, 3), dtype=torch.complex128, device="cuda")
+    for comp in (0, 2, 4, 6):
+        p, q = divmod(comp, 3)
+        unit = zero9.clone()
+        unit[:, p, q] = 1.0
+        dM = plan.assemble_a(plan.material(unit), plan.material(zero3, inverse=True), k0, "fp64")
+        ref = (dM * G).reshape(T, 400).sum(dim=1)
+        assert torch.linalg.norm(e.grad[:, p, q] - ref) <= 1e-11 * torch.linalg.norm(ref)
+    for d in range(3):
+        unit = zero3.clone()
+        unit[:, d] = 1.0
+        dK = plan.assemble_a(plan.material(zero9), plan.material(unit, inverse=True), k0, "fp64")
+        ref = (dK * G).reshape(T, 400).sum(dim=1)
+        assert torch.linalg.norm(m.grad[:, d] - ref) <= 1e-11 * torch.linalg.norm(ref), d

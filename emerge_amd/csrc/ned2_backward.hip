@@ -64,6 +64,39 @@ __device__ __forceinline__ void pullback(const double A[9], const double Wre[9],
         }
 }
 
+// The same as pullback in two steps, O = A^T (W A) * scale, with 18 live temporaries.
+__device__ __forceinline__ void pullback2(const double A[9], const double Wre[9],
+                                          const double Wim[9], double scale, double Ore[9],
+                                          double Oim[9]) {
+    double Tre[9], Tim[9];  // T[p][s] = sum_q W[p][q] A[q][s]
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                re += Wre[3 * p + q] * A[3 * q + c];
+                im += Wim[3 * p + q] * A[3 * q + c];
+            }
+            Tre[3 * p + c] = re;
+            Tim[3 * p + c] = im;
+        }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                re += A[3 * p + r] * Tre[3 * p + c];
+                im += A[3 * p + r] * Tim[3 * p + c];
+            }
+            Ore[3 * r + c] = re * scale;
+            Oim[3 * r + c] = im * scale;
+        }
+}
+
 // Full 3x3 dL/dG (row-major, re/im) from the gradients of the symmetric components
 // (00,11,22,01,02,12) and antisymmetric components (01,02,12) of the split.
 __device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool full,
@@ -101,7 +134,6 @@ __device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool
 // tets' geometry then apply the chain rule.
 constexpr int kWTile = 64;     // tets per wavefront tile (one lane per tet for geometry)
 constexpr int kWChunk = 16;    // tets per chain-rule batch
-constexpr int kWWaves = 4;     // wavefronts per workgroup
 constexpr int kWComps = 3;     // components per lane (per 16-lane row)
 constexpr int kWPairs = 14;    // pairs per lane (15 x 14 = 210)
 constexpr int kWStride = 15;   // LDS slots per lane range (one pad: no bank conflicts)
@@ -111,6 +143,7 @@ template <bool FULL>
 struct WaveCfg {
     static constexpr int AHEAD = 1;               // upstream blocks staged per group
     static constexpr int NRES = FULL ? 18 : 12;   // K sym, M sym (, K anti, M anti)
+    static constexpr int WAVES = FULL ? 2 : 4;    // wavefronts per workgroup
 };
 
 // Scalar/diagonal: 13.3 KB per wavefront (two 4-wavefront workgroups per CU, two
@@ -237,9 +270,8 @@ struct LaneGeom {
 template <bool FULL>
 __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned char (*slot)[20],
                                            const double (&coef)[kWComps][kWPairs],
-                                           const double4* coefa,
-                                           const int (&pa)[4], const int (&pb)[4],
-                                           const int (&pw)[4], const LaneGeom& gm, int lane,
+                                           const double* coefa,
+                                           const int (&pab)[4], const LaneGeom& gm, int lane,
                                            int row, int j, int lt0, int nq, int ntile,
                                            int eps_kind, int mu_kind,
                                            double2* __restrict__ g_eps,
@@ -255,7 +287,8 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
         for (int k = 0; k < 4; ++k) {
             const int p = lane + 64 * k;
             if (p < NED2_NPAIR) {
-                const int a = pa[k], b = pb[k];
+                const int a = pab[k] & 0xff, b = pab[k] >> 8;
+                const int pw = (p / kWPairs) * kWStride + p % kWPairs;  // LDS slot
                 const int ra = slot[code][a], rb = slot[code][b];
                 double2 v = blk[ra * 20 + rb];
                 if (ra != rb) {
@@ -264,12 +297,12 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
                     v.y += v2.y;
                 }
                 const double sgp = (a >= 6 ? s : 1.0) * (b >= 6 ? s : 1.0);
-                sm.w[q][pw[k]] = make_double2(sgp * v.x, sgp * v.y);
+                sm.w[q][pw] = make_double2(sgp * v.x, sgp * v.y);
                 if constexpr (FULL) {
                     // antisymmetric sign: negative when the pair's lower canonical function
                     // sits in the lower reference slot
                     const double sa = ra < rb ? -sgp : sgp;
-                    sm.wa[q][pw[k]] = make_double2(sa * v.x, sa * v.y);
+                    sm.wa[q][pw] = make_double2(sa * v.x, sa * v.y);
                 }
             }
         }
@@ -297,9 +330,9 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
                 // antisymmetric pass: rows 0 and 1 (K, M); rows 2 and 3 repeat them and
                 // their sums are not used. Lane 15's weights are zero.
                 const double2 va = sm.wa[q][j * kWStride + k];
-                const double4 ca =
-                    coefa[(row & 1) * NED2_NPAIR + min(kWPairs * j + k, NED2_NPAIR - 1)];
-                const double cc[3] = {ca.x, ca.y, ca.z};
+                const double* ca =
+                    coefa + 3 * ((row & 1) * NED2_NPAIR + min(kWPairs * j + k, NED2_NPAIR - 1));
+                const double cc[3] = {ca[0], ca[1], ca[2]};
 #pragma unroll
                 for (int c = 0; c < kWComps; ++c) {
                     acca[q][c][0] += va.x * cc[c];
@@ -356,7 +389,7 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
                     wa[2 * c + 1] = r[12 + c].y;
                 }
                 unsplit(ws, wa, true, Wre, Wim);
-                pullback(gm.Jc, Wre, Wim, gm.iad, Ore, Oim);
+                pullback2(gm.Jc, Wre, Wim, gm.iad, Ore, Oim);
                 write_leaf(g_mu, gm.tet, mu_kind, Ore, Oim);
             }
             if (g_eps) {
@@ -373,7 +406,7 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
                 double C[9];
                 cofactors(gm.Jc, C);
                 unsplit(ws, wa, true, Wre, Wim);
-                pullback(C, Wre, Wim, gm.iad, Ore, Oim);
+                pullback2(C, Wre, Wim, gm.iad, Ore, Oim);
                 write_leaf(g_eps, gm.tet, eps_kind, Ore, Oim);
             }
         }
@@ -382,30 +415,31 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
 }
 
 template <bool FULL>
-__global__ __launch_bounds__(64 * kWWaves)
-__attribute__((amdgpu_waves_per_eu(FULL ? 1 : 2, FULL ? 8 : 2)))
+__global__ __launch_bounds__(64 * WaveCfg<FULL>::WAVES)
+__attribute__((amdgpu_waves_per_eu(2, 2)))
 void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, double k0sq,
                                int eps_kind, int mu_kind, double2* __restrict__ g_eps,
                                double2* __restrict__ g_mu) {
     constexpr int AH = WaveCfg<FULL>::AHEAD;
-    __shared__ WaveSmem<FULL> smem[kWWaves];
+    constexpr int NW = WaveCfg<FULL>::WAVES;
+    __shared__ WaveSmem<FULL> smem[NW];
     __shared__ unsigned char slot[64][20];
-    // antisymmetric coefficients (K; M times -k0^2), one padded row per pair
-    __shared__ double4 coefa[FULL ? 2 : 1][FULL ? NED2_NPAIR : 1];
-    for (int i = threadIdx.x; i < 64 * 20; i += 64 * kWWaves)
+    // antisymmetric coefficients (K; M times -k0^2), three per pair
+    __shared__ double coefa[FULL ? 2 : 1][FULL ? NED2_NPAIR : 1][3];
+    for (int i = threadIdx.x; i < 64 * 20; i += 64 * NW)
         slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
     if constexpr (FULL) {
-        for (int i = threadIdx.x; i < 2 * NED2_NPAIR; i += 64 * kWWaves) {
+        for (int i = threadIdx.x; i < 2 * NED2_NPAIR; i += 64 * NW) {
             const int r = i / NED2_NPAIR, p = i % NED2_NPAIR;
-            coefa[r][p] = r == 0 ? make_double4(TANTI_K[p][0], TANTI_K[p][1], TANTI_K[p][2], 0.0)
-                                 : make_double4(-k0sq * TANTI_M[p][0], -k0sq * TANTI_M[p][1],
-                                                -k0sq * TANTI_M[p][2], 0.0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                coefa[r][p][c] = r == 0 ? TANTI_K[p][c] : -k0sq * TANTI_M[p][c];
         }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     WaveSmem<FULL>& sm = smem[wave];
-    const long long tile0 = ((long long)blockIdx.x * kWWaves + wave) * kWTile;
+    const long long tile0 = ((long long)blockIdx.x * NW + wave) * kWTile;
     if (tile0 >= m.n_assy) return;  // whole wavefront
     const int ntile = (int)min((long long)kWTile, m.n_assy - tile0);
     // the upstream blocks of the next AH tets, in flight while the current ones run
@@ -433,13 +467,11 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
             const double v = comp < 6 ? TSYM_K[p][comp] : -k0sq * TSYM_M[p][comp - 6];
             coef[c][k] = j < 15 ? v : 0.0;
         }
-    int pa[4], pb[4], pw[4];
+    int pab[4];  // the functions of the lane's pairs, a | b << 8
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int p = min(lane + 64 * k, NED2_NPAIR - 1);
-        pa[k] = PAIR_A[p];
-        pb[k] = PAIR_B[p];
-        pw[k] = (p / kWPairs) * kWStride + p % kWPairs;
+        pab[k] = PAIR_A[p] | PAIR_B[p] << 8;
     }
     // lane 15's range (read with zero coefficients) is never written by the pair phase
 #pragma unroll
@@ -476,7 +508,7 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
                 }
             }
         wave_sync();
-        wave_group<FULL>(sm, slot, coef, &coefa[0][0], pa, pb, pw, gm, lane, row, j, lt0,
+        wave_group<FULL>(sm, slot, coef, &coefa[0][0][0], pab, gm, lane, row, j, lt0,
                          min(AH, ntile - lt0), ntile, eps_kind, mu_kind, g_eps, g_mu);
     }
 }
@@ -488,13 +520,16 @@ hipError_t launch_backward(const MeshDev& m, const double2* grad, double k0sq, i
     if (m.n_assy <= 0) return hipSuccess;
     const bool full = (g_eps && eps_kind == EM_MAT_FULL) || (g_mu && mu_kind == EM_MAT_FULL);
     const long long wtiles = (m.n_assy + kWTile - 1) / kWTile;
-    const unsigned blocks = (unsigned)((wtiles + kWWaves - 1) / kWWaves);
     if (full) {
-        hipLaunchKernelGGL(ned2_backward_wave_kernel<true>, dim3(blocks), dim3(64 * kWWaves), 0,
-                           stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+        constexpr int nw = WaveCfg<true>::WAVES;
+        hipLaunchKernelGGL(ned2_backward_wave_kernel<true>, dim3((unsigned)((wtiles + nw - 1) / nw)),
+                           dim3(64 * nw), 0, stream, m, grad, k0sq, eps_kind, mu_kind, g_eps,
+                           g_mu);
     } else {
-        hipLaunchKernelGGL(ned2_backward_wave_kernel<false>, dim3(blocks), dim3(64 * kWWaves), 0,
-                           stream, m, grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
+        constexpr int nw = WaveCfg<false>::WAVES;
+        hipLaunchKernelGGL(ned2_backward_wave_kernel<false>,
+                           dim3((unsigned)((wtiles + nw - 1) / nw)), dim3(64 * nw), 0, stream, m,
+                           grad, k0sq, eps_kind, mu_kind, g_eps, g_mu);
     }
     return hipGetLastError();
 }

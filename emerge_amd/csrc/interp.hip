@@ -61,6 +61,48 @@ __global__ __launch_bounds__(256) void cell_fill_kernel(const double* __restrict
     }
 }
 
+// Bounding box of the query points: per-block min/max (NaN coordinates ignored), then one
+// block folds the partials; out6 = (xmin, xmax, ymin, ymax, zmin, zmax).
+constexpr int kBoxBlocks = 1024;
+
+__global__ __launch_bounds__(256) void bbox_partial_kernel(const double* __restrict__ c,
+                                                           long long n,
+                                                           double* __restrict__ part) {
+    __shared__ double red[6][256];
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double x = c[q * n + j];
+            v[2 * q] = fmin(v[2 * q], x);
+            v[2 * q + 1] = fmax(v[2 * q + 1], x);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                red[k][threadIdx.x] = (k & 1) ? fmax(red[k][threadIdx.x], red[k][threadIdx.x + w])
+                                              : fmin(red[k][threadIdx.x], red[k][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void bbox_final_kernel(const double* __restrict__ part, int nb,
+                                                        double* __restrict__ out6) {
+    if (threadIdx.x >= 6) return;
+    const int k = threadIdx.x;
+    double v = (k & 1) ? -INFINITY : INFINITY;
+    for (int b = 0; b < nb; ++b) v = (k & 1) ? fmax(v, part[b * 6 + k]) : fmin(v, part[b * 6 + k]);
+    out6[k] = v;
+}
+
 __global__ __launch_bounds__(256) void fill_i64_kernel(long long* __restrict__ p, long long n,
                                                        long long v) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -263,27 +305,28 @@ hipError_t locate_points(const double4* xyz, const int4* tets, long long n_tets,
     double hc[6];
     Grid g;
     // bounding box of the points on the host side of the copy (one pass, small)
-    double* h = nullptr;
+    double* box = nullptr;  // kBoxBlocks x 6 partials, then the 6 extremes
 #define TRY(x)                          \
     do {                                \
         e = (x);                        \
         if (e != hipSuccess) goto done; \
     } while (0)
     {
-        // min/max per axis: copy the coordinates back once (3 n doubles); the query sets
-        // here are port faces and cut planes, so this stays small next to the search
-        h = new double[3 * n];
-        TRY(hipMemcpyAsync(h, c, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+        // bounding box on the device; only the 6 extremes come back to the host
+        const long long nbl = (n + 255) / 256;
+        const int nb = (int)(nbl < kBoxBlocks ? nbl : kBoxBlocks);
+        TRY(hipMalloc((void**)&box, sizeof(double) * (6 * (size_t)nb + 6)));
+        hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(256), 0, s, c, n, box);
+        TRY(hipGetLastError());
+        hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, s, box, nb, box + 6 * nb);
+        TRY(hipGetLastError());
+        TRY(hipMemcpyAsync(hc, box + 6 * nb, sizeof(double) * 6, hipMemcpyDeviceToHost, s));
         TRY(hipStreamSynchronize(s));
-        for (int q = 0; q < 3; ++q) {
-            hc[2 * q] = h[q * n];
-            hc[2 * q + 1] = h[q * n];
-            for (long long j = 1; j < n; ++j) {
-                const double x = h[q * n + j];
-                hc[2 * q] = x < hc[2 * q] ? x : hc[2 * q];
-                hc[2 * q + 1] = x > hc[2 * q + 1] ? x : hc[2 * q + 1];
+        for (int q = 0; q < 3; ++q)
+            if (!(hc[2 * q] <= hc[2 * q + 1])) {  // no finite coordinate on this axis
+                hc[2 * q] = 0.0;
+                hc[2 * q + 1] = 0.0;
             }
-        }
         int res = (int)cbrt((double)n / 8.0);
         res = res < 20 ? 20 : (res > 200 ? 200 : res);
         const double pad = 1e-3;
@@ -322,7 +365,7 @@ hipError_t locate_points(const double4* xyz, const int4* tets, long long n_tets,
     }
 done:
 #undef TRY
-    delete[] h;
+    (void)hipFree(box);
     (void)hipFree(cnt);
     (void)hipFree(off);
     (void)hipFree(contents);

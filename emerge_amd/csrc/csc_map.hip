@@ -157,26 +157,39 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
     long long* __restrict__ ucount, const long long* __restrict__ indptr,
     long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg,
     unsigned short* __restrict__ pos) {
-    __shared__ unsigned long long key[kColCap];
     __shared__ unsigned long long srt[kColCap];
     const int lane = threadIdx.x;
     for (long long c = blockIdx.x; c < N; c += gridDim.x) {
         const long long s = inc_ptr[c];
         const int n = (int)(NDOF * (inc_ptr[c + 1] - s));
-        for (int i = lane; i < n; i += 64) {
-            const int e = inc[s + i / NDOF];
-            const long long ia = e / NDOF;
-            const int row = t2f[tet_of(tetids, ia) * NDOF + (i % NDOF)];
-            key[i] = ((unsigned long long)(unsigned)row << 32) | (unsigned)i;
+        int npad = 1;
+        while (npad < n) npad <<= 1;
+        for (int i = lane; i < npad; i += 64) {
+            unsigned long long k = ~0ull;  // padding sorts last
+            if (i < n) {
+                const int e = inc[s + i / NDOF];
+                const long long ia = e / NDOF;
+                const int row = t2f[tet_of(tetids, ia) * NDOF + (i % NDOF)];
+                k = ((unsigned long long)(unsigned)row << 32) | (unsigned)i;
+            }
+            srt[i] = k;
         }
         __syncthreads();
-        for (int i = lane; i < n; i += 64) {
-            const unsigned long long ki = key[i];
-            int r = 0;
-            for (int j = 0; j < n; ++j) r += key[j] < ki;
-            srt[r] = ki;
-        }
-        __syncthreads();
+        // bitonic sort of the (row, local index) keys, all distinct, in LDS
+        for (int k = 2; k <= npad; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = lane; i < npad; i += 64) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned long long a = srt[i], b = srt[ixj];
+                        if ((a > b) == ((i & k) == 0)) {
+                            srt[i] = b;
+                            srt[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
         long long carry = 0;
         for (int base = 0; base < n; base += 64) {
             const int i = base + lane;

@@ -267,6 +267,20 @@ def suite_csc_pec(args) -> dict:
     same_inc = bool(torch.equal(
         cm.to_csc_combine(K, M, k0, block_symmetric=True),
         cm.to_csc_combine(K, M, k0, block_symmetric=True, segment_gather=True)))
+    # blocks that are not symmetric (full anisotropic tensors) cannot use the mirrored
+    # reads; the staged gather writes the contributions block-transposed first
+    work = torch.empty(plan.nnz_coo, dtype=torch.complex128, device="cuda")
+    ms_staged, _ = time_launches(lambda: cm.to_csc_staged(K, M, k0, work=work, out=full), 10, 2)
+    same_staged = bool(torch.equal(cm.to_csc_staged(K, M, k0, work=work),
+                                   cm.to_csc_combine(K, M, k0)))
+    del work
+    # complex128 (the reference's dtype): the same gathers on FP64 K and M
+    K64, M64 = plan.assemble_km(eps, mu, "fp64")
+    full64 = torch.empty(cm.nnz, dtype=torch.complex128, device="cuda")
+    ms64_m, _ = time_launches(
+        lambda: cm.to_csc_combine(K64, M64, k0, out=full64, block_symmetric=True), 10, 2)
+    ms64, _ = time_launches(lambda: cm.to_csc_combine(K64, M64, k0, out=full64), 10, 2)
+    del K64, M64, full64
     ref_blk = select_csc(ex["indptr"], ex["indices"], cm.to_csc_combine(K, M, k0), sel)[2]
     sub.to_csc_combine(K, M, k0, out=blk)
     torch.cuda.synchronize()
@@ -297,21 +311,29 @@ def suite_csc_pec(args) -> dict:
                                 "to_csc_combine_full_mirrored": ms_full_m,
                                 "restricted_to_csc_combine_mirrored": ms_sub_m,
                                 "to_csc_combine_full_segment_gather": ms_full_seg,
-                                "to_csc_combine_full_segment_gather_mirrored": ms_full_seg_m},
+                                "to_csc_combine_full_segment_gather_mirrored": ms_full_seg_m,
+                                "to_csc_combine_full_staged": ms_staged},
+           "staged_equals_plain": same_staged,
            "incidence_equals_segment_gather": same_inc,
            "restricted_equals_selected_full": same,
            "mirrored_equals_plain": same_m,
            "cpu_scipy_select_ms": cpu_ms,
            "block_coo_contributors": n_contrib,
-           "full_gather_gb_per_s": (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full * 1e-3) / 1e9,
-           "restricted_gather_gb_per_s": (n_contrib * 24 + sub.nnz * 16) / (ms_sub * 1e-3) / 1e9,
-           "full_gather_mirrored_gb_per_s":
-               (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full_m * 1e-3) / 1e9,
-           "note": "gb_per_s figures use the segment gather's algorithmic bytes (perm 8 + "
-                   "E, B 8 + 8 per contributor, seg 8 + out 8 per entry); the incidence "
-                   "gather moves 2 (pos) + 16 per contributor and 8 per entry",
-           "restricted_gather_mirrored_gb_per_s":
-               (n_contrib * 24 + sub.nnz * 16) / (ms_sub_m * 1e-3) / 1e9}
+           "fp64_per_frequency_ms": {"to_csc_combine_full_mirrored": ms64_m,
+                                     "to_csc_combine_full": ms64},
+           # algorithmic bytes of the incidence gather: per COO contributor its row
+           # position (2) + E and B (8 + 8 complex64, 16 + 16 complex128), per CSC entry
+           # the value out (8 / 16)
+           "full_mirrored_gb_per_s":
+               (plan.nnz_coo * 18 + cm.nnz * 8) / (ms_full_m * 1e-3) / 1e9,
+           "full_mirrored_fp64_gb_per_s":
+               (plan.nnz_coo * 34 + cm.nnz * 16) / (ms64_m * 1e-3) / 1e9,
+           "restricted_mirrored_gb_per_s":
+               (n_contrib * 18 + sub.nnz * 8) / (ms_sub_m * 1e-3) / 1e9,
+           # the segment gather's: perm 8 + E, B 8 + 8 per contributor, seg 8 + out 8 per
+           # entry
+           "full_segment_gather_gb_per_s":
+               (plan.nnz_coo * 24 + cm.nnz * 16) / (ms_full_seg * 1e-3) / 1e9}
     del full, blk, K, M, cm, sub
     torch.cuda.empty_cache()
     return res

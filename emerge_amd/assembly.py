@@ -400,6 +400,34 @@ class CscMap:
                                           _stream_ptr(stream)))
         return out
 
+    def to_csc_staged(self, dataE: torch.Tensor, dataB: torch.Tensor | None, k0: float,
+                      work: torch.Tensor | None = None, out=None,
+                      stream=None) -> torch.Tensor:
+        """CSC values of ``dataE - k0**2 * dataB`` (``dataB`` may be None) for blocks that
+        are not symmetric (full anisotropic tensors): the contributions are staged
+        block-transposed in FP64 in ``work`` (complex128, ``nnz_coo`` entries; allocated
+        when None) and summed with contiguous row reads. Equal bit for bit to
+        ``to_csc_combine`` / ``to_csc``."""
+        if dataE.numel() != self.nnz_coo or dataE.dtype not in (torch.complex128, torch.complex64):
+            raise ValueError(f"COO data must be complex with {self.nnz_coo} entries")
+        if dataB is not None and (dataB.dtype != dataE.dtype or dataB.numel() != dataE.numel()):
+            raise ValueError("dataE and dataB must match")
+        prec = PREC_FP64 if dataE.dtype == torch.complex128 else PREC_MIXED
+        dataE = dataE.contiguous()
+        dataB = dataB.contiguous() if dataB is not None else None
+        if work is None:
+            work = torch.empty(self.nnz_coo, dtype=torch.complex128, device=self.device)
+        _check_out(work, self.nnz_coo, torch.complex128)
+        if out is None:
+            out = torch.empty(self.nnz, dtype=dataE.dtype, device=self.device)
+        _check_out(out, self.nnz, dataE.dtype)
+        check(_lib.lib().em_csc_to_csc_staged(
+            self._h, ctypes.c_void_p(dataE.data_ptr()),
+            None if dataB is None else ctypes.c_void_p(dataB.data_ptr()), float(k0),
+            ctypes.c_void_p(work.data_ptr()), ctypes.c_void_p(out.data_ptr()), prec,
+            _stream_ptr(stream)))
+        return out
+
 
 def _check_out(t: torch.Tensor, numel: int, dt) -> None:
     if t.dtype != dt or not t.is_cuda or t.numel() != numel:

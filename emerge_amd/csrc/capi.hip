@@ -714,6 +714,31 @@ int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, dou
     return csc_apply(map, dataE, dataB, k0, data_csc, precision, flags, stream);
 }
 
+int em_csc_to_csc_staged(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                         void* work, void* data_csc, int32_t precision, void* stream) {
+    if (!map) {
+        set_error("null csc map");
+        return EM_EINVAL;
+    }
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("unknown precision");
+        return EM_EINVAL;
+    }
+    if (map->b.nnz > 0 && (!dataE || !data_csc || !work)) {
+        set_error("to_csc_staged: null data, output or workspace");
+        return EM_EINVAL;
+    }
+    if (map->b.ndof != 20 || !map->b.inc || !map->b.pos) {
+        set_error("to_csc_staged: needs a Nedelec (20-DOF) map with incidence lists");
+        return EM_EUNSUPPORTED;
+    }
+    DeviceGuard g(map->device);
+    EM_HIP(em::csc_gather_inc_staged(map->b, map->N, map->nnz_coo / 400, dataE, dataB, k0 * k0,
+                                     (double2*)work, data_csc,
+                                     precision == EM_PREC_MIXED ? 1 : 0, (hipStream_t)stream));
+    return EM_OK;
+}
+
 // ---------------------------------------------------------------------------------------
 // PEC elimination and the solver hand-off
 // ---------------------------------------------------------------------------------------

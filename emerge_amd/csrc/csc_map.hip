@@ -302,12 +302,12 @@ constexpr int kIncCap = 128;
 constexpr int kIncWaves = 4;  // wavefronts per workgroup
 constexpr int kIncCols = 3;   // columns per wavefront
 
-template <typename T, bool MIRROR, bool HAS_B>
+template <typename T, bool MIRROR, bool HAS_B, typename TO = T>
 __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
     const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
     const int* __restrict__ inc, const unsigned short* __restrict__ pos,
     const long long* __restrict__ cols, long long N, const T* __restrict__ E,
-    const T* __restrict__ B, double k0sq, T* __restrict__ out) {
+    const T* __restrict__ B, double k0sq, TO* __restrict__ out) {
     __shared__ double acc[kIncWaves * kIncCols][kIncCap][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane / 20, ii = lane - 20 * grp;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
                         atomicAdd(&a[p[u]][1], v[u].y);
                     }
             }
-            for (int p = ii; p < wl; p += 20) store_c<T>(out, cb + w0 + p,
+            for (int p = ii; p < wl; p += 20) store_c<TO>(out, cb + w0 + p,
                                                          make_double2(a[p][0], a[p][1]));
         }
     }
@@ -644,6 +644,79 @@ static void launch_inc(const CscBuild& b, long long N, const void* E, const void
         hipLaunchKernelGGL((csc_gather_inc_kernel<T, MIRROR, false>), g, blk, 0, s, b.indptr,
                            b.inc_ptr, b.inc, b.pos, b.cols, N, (const T*)E, (const T*)B, k0sq,
                            (T*)out);
+}
+
+// Staging for blocks that are not symmetric: work[400 ia + 20 jj + ii] = the FP64
+// contribution E - k0sq B of entry (ii, jj) (the value the plain gather would form), one
+// wavefront per block through LDS, so that the mirrored gather over work reads each
+// incident block's row contiguously and sums the same values in the same order.
+template <typename T, bool HAS_B>
+__global__ __launch_bounds__(256) void coo_stage_transpose_kernel(const T* __restrict__ E,
+                                                                  const T* __restrict__ B,
+                                                                  double k0sq, long long n_blk,
+                                                                  double2* __restrict__ work) {
+    __shared__ double2 tile[4][400];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (long long blk = (long long)blockIdx.x * 4 + wave; blk < n_blk;
+         blk += (long long)gridDim.x * 4) {
+        const long long base = blk * 400;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int e = lane + 64 * k;
+            if (e < 400) {
+                const int ii = e / 20, jj = e - 20 * ii;
+                tile[wave][20 * jj + ii] = contrib<T>(E, HAS_B ? B : nullptr, k0sq, base + e);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const int e = lane + 64 * k;
+            if (e < 400) work[base + e] = tile[wave][e];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+hipError_t csc_gather_inc_staged(const CscBuild& b, long long N, long long n_blk, const void* E,
+                                 const void* B, double k0sq, double2* work, void* out,
+                                 int complex64, hipStream_t s) {
+    if (N == 0 || b.nnz == 0) return hipSuccess;
+    const long long gb = (n_blk + 3) / 4;
+    const dim3 gt((unsigned)(gb < 65536 * 4 ? gb : 65536 * 4));
+    if (complex64) {
+        if (B)
+            hipLaunchKernelGGL((coo_stage_transpose_kernel<float2, true>), gt, dim3(256), 0, s,
+                               (const float2*)E, (const float2*)B, k0sq, n_blk, work);
+        else
+            hipLaunchKernelGGL((coo_stage_transpose_kernel<float2, false>), gt, dim3(256), 0, s,
+                               (const float2*)E, (const float2*)B, k0sq, n_blk, work);
+    } else {
+        if (B)
+            hipLaunchKernelGGL((coo_stage_transpose_kernel<double2, true>), gt, dim3(256), 0, s,
+                               (const double2*)E, (const double2*)B, k0sq, n_blk, work);
+        else
+            hipLaunchKernelGGL((coo_stage_transpose_kernel<double2, false>), gt, dim3(256), 0, s,
+                               (const double2*)E, (const double2*)B, k0sq, n_blk, work);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long long per_block = kIncWaves * kIncCols;
+    const long long nb = (N + per_block - 1) / per_block;
+    const dim3 g((unsigned)(nb < (1 << 20) ? nb : (1 << 20))), blk(64 * kIncWaves);
+    if (complex64)
+        hipLaunchKernelGGL((csc_gather_inc_kernel<double2, true, false, float2>), g, blk, 0, s,
+                           b.indptr, b.inc_ptr, b.inc, b.pos, b.cols, N, (const double2*)work,
+                           (const double2*)nullptr, 0.0, (float2*)out);
+    else
+        hipLaunchKernelGGL((csc_gather_inc_kernel<double2, true, false, double2>), g, blk, 0, s,
+                           b.indptr, b.inc_ptr, b.inc, b.pos, b.cols, N, (const double2*)work,
+                           (const double2*)nullptr, 0.0, (double2*)out);
+    return hipGetLastError();
 }
 
 hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,

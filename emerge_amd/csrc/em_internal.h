@@ -108,6 +108,11 @@ hipError_t heat_assemble(const double4* xyz, const int4* tets, long long T, cons
 hipError_t heat_rowcol(const int* t2f, long long T, long long* rows, long long* cols,
                        hipStream_t s);
 // to_csc by column incidences (full maps: inc and pos present); same result as csc_gather.
+// Plain (non-symmetric) incidence gather through a block-transposed FP64 staging copy
+// (work: complex128 [400 n_blk]); the same result bit for bit.
+hipError_t csc_gather_inc_staged(const CscBuild& b, long long N, long long n_blk, const void* E,
+                                 const void* B, double k0sq, double2* work, void* out,
+                                 int complex64, hipStream_t s);
 hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,
                           double k0sq, void* out, int complex64, int mirror, hipStream_t s);
 // Exclusive scan of n int64 values (in may equal out); allocates temporaries, blocks.

@@ -183,14 +183,24 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
  */
 #define EM_CSC_BLOCK_SYMMETRIC 1
 /*
- * A full map sums each column from its tet incidences (one wavefront per column, values
- * read row by row from the blocks); EM_CSC_SEGMENT_GATHER selects the per-entry gather
- * through perm/seg instead (what restricted maps use). Both give the same result bit for
- * bit; the flag exists to compare them.
+ * A map sums each column from its tet incidences (20 lanes per column, three columns per
+ * wavefront, in COO order); EM_CSC_SEGMENT_GATHER selects the per-entry gather through
+ * perm/seg instead. Both give the same result bit for bit; the flag exists to compare
+ * them.
  */
 #define EM_CSC_SEGMENT_GATHER 2
 int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                      void* data_csc, int32_t precision, int32_t flags, void* stream);
+/*
+ * to_csc of blocks that are not symmetric (full anisotropic tensors) without the plain
+ * gather's strided column reads: the contributions dataE - k0^2 dataB (dataB may be NULL)
+ * are first written block-transposed in FP64 to the caller's workspace work (complex128,
+ * nnz_coo entries of the map; em_csc_map_sizes), then summed by the mirrored column
+ * gather. The result equals em_csc_to_csc_ex's bit for bit. No counterpart in the
+ * reference (same operation as CSCMapping.to_csc, csc_cast.py:55-63, 162-177).
+ */
+int em_csc_to_csc_staged(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
+                         void* work, void* data_csc, int32_t precision, void* stream);
 
 /*
  * PEC elimination and the solver hand-off. All pointers are DEVICE pointers; functions

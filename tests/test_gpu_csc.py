@@ -183,3 +183,33 @@ def test_incidence_gather_equals_segment_gather(delaunay_field, mesh, precision)
                                                                 local_frame=True)
         ref = oracle.CSCMap(rows, cols, field.n_field).to_csc_data(dE - k0 ** 2 * dB)
         assert rel_fro(cm.to_csc_combine(K, M, k0).cpu().numpy(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("mesh", ["delaunay", "fan30"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_staged_gather_non_symmetric_blocks(delaunay_field, mesh, precision):
+    """COO values whose blocks are not symmetric (random data: the assembly itself mirrors
+    its upper triangle, curlcurl.py:358-376, so its blocks are): the staged gather
+    (contributions written block-transposed in FP64, then the mirrored column sum) equals
+    the plain incidence gather and the segment gather bit for bit, for to_csc and the
+    combine, on the full map and on a restricted (free-DOF) map."""
+    import torch
+    from emerge_amd.assembly import CscMap
+    from emerge_amd.pec import pec_solve_ids
+    field = delaunay_field if mesh == "delaunay" else _fan_field(30)
+    plan = _field_plan(field)
+    cm = CscMap(plan)
+    k0 = 2 * np.pi * 9e9 / 299792458.0
+    dt = torch.complex128 if precision == "fp64" else torch.complex64
+    g = torch.Generator().manual_seed(31)
+    K = torch.randn(cm.nnz_coo, generator=g, dtype=dt).cuda()
+    M = torch.randn(cm.nnz_coo, generator=g, dtype=dt).cuda()
+    work = torch.empty(cm.nnz_coo, dtype=torch.complex128, device="cuda")
+    ref = cm.to_csc_combine(K, M, k0, segment_gather=True)
+    assert torch.equal(cm.to_csc_staged(K, M, k0, work=work), ref)
+    assert torch.equal(cm.to_csc_combine(K, M, k0), ref)
+    assert torch.equal(cm.to_csc_staged(K, None, 0.0, work=work), cm.to_csc(K, segment_gather=True))
+    # restricted map: the free-DOF block of an all-PEC box
+    sel = pec_solve_ids(plan, field, [], field.mesh.boundary_tris())
+    sub = cm.restrict(sel)
+    assert torch.equal(sub.to_csc_staged(K, M, k0, work=work), sub.to_csc_combine(K, M, k0))

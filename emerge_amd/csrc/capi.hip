@@ -225,10 +225,11 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
     // node/material reads hit that XCD's L2: tile j*kXcds + x is the j-th tile of XCD x,
     // which takes the windows w with w % kXcds == x (shorter lists padded with empty tiles).
     // Small windows keep the blocks written at the same time close together in memory:
-    // on the 2.8M-tet bench mesh 512-tet windows launch 6% faster than 16,384-tet ones
-    // (mixed 1.59 vs 1.69 ms; FP64 3.17 vs 3.33 ms), 256-tet windows are slower again
-    // (tiles of a few tets each).
-    constexpr long long kWindow = 512;
+    // on a 2.8M-tet mesh 1,024-tet windows launch 4% faster than 16,384-tet ones (mixed
+    // 1.63 vs 1.69 ms) and 512-tet windows 6%; but on an 8.3M-tet mesh tiles of ~20 tets
+    // (512-tet windows, or 32-tet tiles) are 30% slower than tiles of 40-130 tets, which
+    // 1,024-tet windows keep (5.15 ms vs 5.31 ms with 16,384).
+    constexpr long long kWindow = 1024;
     constexpr int kXcds = 8;
     std::vector<int> horder((size_t)n_assy);
     std::vector<std::vector<int2>> htiles(3);

@@ -295,6 +295,22 @@ def suite_csc_pec(args) -> dict:
     kept_q = keep[ex["indices"]] & keep[col]
     n_contrib = int(kept_q[tgt].sum())
     del tgt, col, kept_q
+    # the oracle's restatement of CSCMapping (pattern, then the per-frequency scatter) on
+    # the first 50k tets' COO entries, one host thread, scaled to the whole mesh
+    import oracle
+    n_cpu = min(T, 50_000)
+    rows_all, cols_all = plan.rowcol()
+    r_h = rows_all[:400 * n_cpu].cpu().numpy()
+    c_h = cols_all[:400 * n_cpu].cpu().numpy()
+    del rows_all, cols_all
+    d_h = (K[:400 * n_cpu] - (k0 ** 2) * M[:400 * n_cpu]).to(torch.complex128).cpu().numpy()
+    t0 = time.perf_counter()
+    ocm = oracle.CSCMap(r_h, c_h, N)
+    cpu_map_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    ocm.to_csc_data(d_h)
+    cpu_csc_ms = (time.perf_counter() - t0) * 1e3
+    del ocm, r_h, c_h, d_h
     A_h = to_scipy(ex["indptr"], ex["indices"], full.to(torch.complex128), N)
     ids = sel.solve_ids.cpu().numpy()
     t0 = time.perf_counter()
@@ -318,6 +334,10 @@ def suite_csc_pec(args) -> dict:
            "restricted_equals_selected_full": same,
            "mirrored_equals_plain": same_m,
            "cpu_scipy_select_ms": cpu_ms,
+           "cpu_oracle_csc": {"sample_tets": n_cpu, "threads": 1,
+                              "map_ms": cpu_map_ms, "to_csc_ms": cpu_csc_ms,
+                              "map_ms_scaled": cpu_map_ms * T / n_cpu,
+                              "to_csc_ms_scaled": cpu_csc_ms * T / n_cpu},
            "block_coo_contributors": n_contrib,
            "fp64_per_frequency_ms": {"to_csc_combine_full_mirrored": ms64_m,
                                      "to_csc_combine_full": ms64},

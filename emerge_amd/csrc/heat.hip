@@ -23,6 +23,12 @@ namespace {
 constexpr int kHeatBlock = 256;
 constexpr int kHeatTile = 64;
 constexpr int kHeatOwners = 250;  // 50 entry pairs x 5 tets per round
+// Workgroups are dispatched round-robin over the 8 XCDs. Workgroup b runs tile
+// ((j / kHeatWindow) * 8 + b % 8) * kHeatWindow + j % kHeatWindow, j = b / 8: each XCD
+// writes runs of kHeatWindow consecutive tiles (1,024 tets; 3% faster on 2.8M tets than
+// tile b on workgroup b, which spreads every XCD's writes over the whole output).
+constexpr int kXcds = 8;
+constexpr int kHeatWindow = 16;
 
 // Per tet of a tile: the conductance terms (16, or 10 when C is symmetric), V, rho_cp V and
 // the orientation, one 160-byte row.
@@ -73,7 +79,12 @@ __global__ __launch_bounds__(kHeatBlock) void heat_assemble_kernel(
     constexpr int NT = NC == 9 ? 16 : 10;
     __shared__ HeatRec rec;
     const int tid = threadIdx.x;
-    const long long t0 = (long long)blockIdx.x * kHeatTile;
+    const long long ntile = (T + kHeatTile - 1) / kHeatTile;
+    const long long jb = blockIdx.x / kXcds;
+    const long long tile =
+        ((jb / kHeatWindow) * kXcds + blockIdx.x % kXcds) * kHeatWindow + jb % kHeatWindow;
+    if (tile >= ntile) return;  // padding of the last window row; the whole workgroup leaves
+    const long long t0 = tile * kHeatTile;
     const int cnt = (int)(T - t0 < kHeatTile ? T - t0 : kHeatTile);
     if (tid < kHeatTile) {
         int sg = 0;
@@ -200,8 +211,10 @@ hipError_t heat_assemble(const double4* xyz, const int4* tets, long long T, cons
                          long long rts, double* K, double* M, hipStream_t s) {
     if (T <= 0) return hipSuccess;
     const long long ntiles = (T + kHeatTile - 1) / kHeatTile;
-    if (ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    const dim3 g((unsigned)ntiles);
+    const long long span = (long long)kXcds * kHeatWindow;
+    const long long nblk = (ntiles + span - 1) / span * span;
+    if (nblk > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 g((unsigned)nblk);
     if (!K) {
         hipLaunchKernelGGL(heat_assemble_kernel<0>, g, dim3(kHeatBlock), 0, s, xyz, tets, T,
                            nullptr, 0LL, 0LL, rho, rts, nullptr, M);

@@ -707,7 +707,8 @@ int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_
 
 int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                           void* data_csc, int32_t precision, void* stream) {
-    if (!dataB) {
+    // a map without entries (no assembled tets) reads nothing: empty inputs may be null
+    if (map && map->b.nnz > 0 && !dataB) {
         set_error("to_csc_combine: null dataB");
         return EM_EINVAL;
     }

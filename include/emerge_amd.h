@@ -169,7 +169,10 @@ int em_csc_map_export(em_csc_map_t map, int64_t* indptr, int64_t* indices, int64
  */
 int em_csc_to_csc(em_csc_map_t map, const void* data_coo, void* data_csc, int32_t precision,
                   void* stream);
-/* Fused to_csc(dataE - k0^2 * dataB) (assembler.py:494) without forming the COO sum. */
+/*
+ * Fused to_csc(dataE - k0^2 * dataB) (assembler.py:494) without forming the COO sum.
+ * Pointers may be NULL only for a map without entries (a plan with no assembled tets).
+ */
 int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                           void* data_csc, int32_t precision, void* stream);
 /*

@@ -22,12 +22,9 @@ def _ref_tensor_diag(T, diag):
     return out
 
 
-@pytest.fixture(scope="module")
-def setup():
+def _setup(field):
     import torch
     from emerge_amd.assembly import Ned2Plan
-    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
-    field = Nedelec2Field(delaunay_box_mesh(3000, seed=17))
     plan = Ned2Plan.from_field(field)
     T, N = field.n_tets, field.n_field
     rng = np.random.default_rng(17)
@@ -42,6 +39,12 @@ def setup():
                 cols=cols, k0=k0, T=T, rng=rng)
 
 
+@pytest.fixture(scope="module")
+def setup():
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+    return _setup(Nedelec2Field(delaunay_box_mesh(3000, seed=17)))
+
+
 def _grad(s, precision="fp64"):
     from emerge_amd.autograd import assemble_a, bilinear_form
     e = s["eps"].clone().requires_grad_(True)
@@ -52,8 +55,7 @@ def _grad(s, precision="fp64"):
     return e.grad.cpu().numpy(), m.grad.cpu().numpy(), float(L.detach())
 
 
-def test_backward_matches_fp64_reference_gradient(setup):
-    s = setup
+def _check_reference_gradient(s):
     f, T, k0 = s["field"], s["T"], s["k0"]
     ge, gm, _ = _grad(s)
     I = _ref_tensor_diag(T, [1, 1, 1])
@@ -70,6 +72,19 @@ def test_backward_matches_fp64_reference_gradient(setup):
                                                 local_frame=True)[0]
         ref_m = ((Kd - K0) * G).reshape(T, 400).sum(axis=1)
         assert np.linalg.norm(gm[:, d] - ref_m) <= 1e-10 * np.linalg.norm(ref_m), d
+
+
+def test_backward_matches_fp64_reference_gradient(setup):
+    _check_reference_gradient(setup)
+
+
+@pytest.mark.parametrize("order", [(0, 1, 2, 3), (1, 0, 2, 3)])
+def test_backward_one_tet(order):
+    """A one-tet plan in both orientations: one wavefront tile holding a single tet."""
+    from emerge_amd.mesh import Nedelec2Field, TetMesh
+    nodes = np.array([[0, 1.0, 0, 0], [0, 0, 1.0, 0], [0, 0, 0, 1.0]]) * 1e-2
+    field = Nedelec2Field(TetMesh(nodes, np.array(order, np.int64).reshape(4, 1)))
+    _check_reference_gradient(_setup(field))
 
 
 def test_backward_matches_central_differences(setup):

@@ -124,3 +124,39 @@ def test_plan_without_assembled_tets(wr90_field):
     assert E.size == 0 and B.size == 0
     S = mapping.to_csc(E)
     assert S.shape == (N, N) and S.nnz == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_output_views_at_16_byte_offsets(wr90_field, precision):
+    """Outputs need 16-byte alignment only: views 16 bytes into larger buffers get the
+    values of freshly allocated outputs and leave the elements around them untouched; a
+    view 8 bytes in (complex64) is refused before any launch."""
+    import torch
+    from emerge_amd._lib import EmError
+    from emerge_amd.assembly import Ned2Plan
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(14)
+    er, ur = ref_tensor(T, "scalar", rng), ref_tensor(T, "diag", rng)
+    plan = Ned2Plan.from_field(f)
+    n = plan.nnz_coo
+    A = plan.assemble_a(er, ur, K0, precision)
+    K, M = plan.assemble_km(er, ur, precision)
+    shift = 1 if precision == "fp64" else 2  # elements in 16 bytes
+    fill = 7.0 + 7.0j
+
+    def padded():
+        return torch.full((n + 2 * shift,), fill, dtype=A.dtype, device="cuda")
+
+    bufs = [padded() for _ in range(3)]
+    views = [b[shift:shift + n] for b in bufs]
+    assert all(v.data_ptr() % 16 == 0 and v.data_ptr() % 32 != 0 for v in views)
+    plan.assemble_a(er, ur, K0, precision, out=views[0])
+    plan.assemble_km(er, ur, precision, out=(views[1], views[2]))
+    assert torch.equal(views[0], A)
+    assert torch.equal(views[1], K) and torch.equal(views[2], M)
+    for b in bufs:
+        assert bool((b[:shift] == fill).all()) and bool((b[shift + n:] == fill).all())
+    if precision == "mixed":
+        with pytest.raises(EmError, match="aligned"):
+            plan.assemble_a(er, ur, K0, precision, out=padded()[1:1 + n])

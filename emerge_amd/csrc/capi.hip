@@ -612,9 +612,9 @@ int em_csc_map_create(em_csc_map_t* map, em_ned2_plan_t plan, void* stream) {
     hipError_t e = em::csc_build(plan->dev(), c->N, c->b, max_inc, (hipStream_t)stream);
     if (e != hipSuccess) {
         delete c;
-        if (e == hipErrorInvalidValue && max_inc * 20 > 2048) {
-            set_error("csc_map_create: a DOF lies in " + std::to_string(max_inc) +
-                      " tets (at most 102 supported)");
+        if (e == hipErrorInvalidValue && max_inc * 20 > em::kCscMaxColumnKeys) {
+            set_error("csc_map_create: a DOF lies in " + std::to_string(max_inc) + " tets (at most " +
+                      std::to_string(em::kCscMaxColumnKeys / 20) + " supported)");
             return EM_EUNSUPPORTED;
         }
         set_error(std::string("csc_map_create: ") + hipGetErrorString(e));
@@ -1162,9 +1162,9 @@ int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream)
                                         (hipStream_t)stream);
     if (e != hipSuccess) {
         delete c;
-        if (e == hipErrorInvalidValue && max_inc * 10 > 2048) {
-            set_error("csc_map_create_heat: a DOF lies in " + std::to_string(max_inc) +
-                      " tets (at most 204 supported)");
+        if (e == hipErrorInvalidValue && max_inc * 10 > em::kCscMaxColumnKeys) {
+            set_error("csc_map_create_heat: a DOF lies in " + std::to_string(max_inc) + " tets (at most " +
+                      std::to_string(em::kCscMaxColumnKeys / 10) + " supported)");
             return EM_EUNSUPPORTED;
         }
         set_error(std::string("csc_map_create_heat: ") + hipGetErrorString(e));

@@ -162,6 +162,7 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
     for (long long c = blockIdx.x; c < N; c += gridDim.x) {
         const long long s = inc_ptr[c];
         const int n = (int)(NDOF * (inc_ptr[c + 1] - s));
+        if (n > kColCap) continue;  // csc_long_column_kernel builds it
         int npad = 1;
         while (npad < n) npad <<= 1;
         for (int i = lane; i < npad; i += 64) {
@@ -214,6 +215,82 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
             carry += __popcll(mask);
         }
         if (!FILL && lane == 0) ucount[c] = carry;
+        __syncthreads();  // LDS reuse by the next column
+    }
+}
+
+// Columns of more than kColCap keys (a DOF in more than 102 Nedelec-2 or 204 Lagrange-2
+// tets: an edge on the axis of a fine fan, a vertex at a sharp corner) are rare. One
+// 256-thread workgroup takes each: the column's rows go to LDS, and every key is ranked
+// against all others (O(n^2) LDS reads). The keys (row, local index) are distinct, so a
+// key's rank is its position in the sorted order, and the outputs equal those of
+// csc_column_kernel's sort. The first key of a row (smallest local index among equal
+// rows) carries the row's entry: its unique position is the number of distinct rows
+// below it. Columns of more than kLongCap keys are refused on the host.
+constexpr int kLongCap = (int)kCscMaxColumnKeys;
+constexpr int kLongThreads = 256;
+static_assert(kLongCap % 64 == 0, "first-key bitmask words");
+
+template <bool FILL, int NDOF>
+__global__ __launch_bounds__(kLongThreads) void csc_long_column_kernel(
+    const int* __restrict__ t2f, const int* __restrict__ tetids,
+    const long long* __restrict__ inc_ptr, const int* __restrict__ inc, long long N,
+    long long* __restrict__ ucount, const long long* __restrict__ indptr,
+    long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg,
+    unsigned short* __restrict__ pos) {
+    __shared__ int rows[kLongCap];
+    __shared__ unsigned long long first[kLongCap / 64];
+    const int tid = threadIdx.x;
+    for (long long c = blockIdx.x; c < N; c += gridDim.x) {
+        const long long s = inc_ptr[c];
+        const long long nl = NDOF * (inc_ptr[c + 1] - s);
+        if (nl <= kColCap || nl > kLongCap) continue;  // same c for the whole workgroup
+        const int n = (int)nl, nw = (n + 63) / 64;
+        for (int i = tid; i < n; i += kLongThreads)
+            rows[i] = t2f[tet_of(tetids, inc[s + i / NDOF] / NDOF) * NDOF + (i % NDOF)];
+        __syncthreads();
+        // i runs over whole 64-key words, so every lane of a wavefront takes the ballot
+        for (int i = tid; i < 64 * nw; i += kLongThreads) {
+            bool f = i < n;
+            if (f) {
+                const int r = rows[i];
+                for (int j = 0; j < i; ++j)
+                    if (rows[j] == r) {
+                        f = false;
+                        break;
+                    }
+            }
+            const unsigned long long m = __ballot(f);
+            if ((tid & 63) == 0) first[i >> 6] = m;
+        }
+        __syncthreads();
+        if (!FILL) {
+            if (tid == 0) {
+                long long u = 0;
+                for (int w = 0; w < nw; ++w) u += __popcll(first[w]);
+                ucount[c] = u;
+            }
+        } else {
+            const long long q0 = indptr[c];
+            for (int i = tid; i < n; i += kLongThreads) {
+                const int r = rows[i];
+                int rank = 0, q = 0;
+                for (int j = 0; j < n; ++j) {
+                    const int rj = rows[j];
+                    rank += (rj < r) || (rj == r && j < i);
+                    q += (rj < r) && ((first[j >> 6] >> (j & 63)) & 1ull);
+                }
+                const int e = inc[s + i / NDOF];
+                const long long ia = e / NDOF;
+                const int jj = e % NDOF, ii = i % NDOF;
+                perm[NDOF * s + rank] = (long long)NDOF * NDOF * ia + NDOF * ii + jj;
+                if (pos) pos[NDOF * s + i] = (unsigned short)q;
+                if ((first[i >> 6] >> (i & 63)) & 1ull) {
+                    indices[q0 + q] = r;
+                    seg[q0 + q] = NDOF * s + rank;
+                }
+            }
+        }
         __syncthreads();  // LDS reuse by the next column
     }
 }
@@ -423,6 +500,7 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
     unsigned long long *cnt = nullptr, *maxlen = nullptr;
     int* inc = nullptr;
     long long* ucount = nullptr;
+    bool long_cols = false;
 #define TRY(x)                          \
     do {                                \
         e = (x);                        \
@@ -460,21 +538,29 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
         TRY(hipMemcpyAsync(&ml, maxlen, sizeof(ml), hipMemcpyDeviceToHost, s));
         TRY(hipStreamSynchronize(s));
         max_inc = (long long)ml;
-        if (NDOF * max_inc > kColCap) {
+        if (NDOF * max_inc > kLongCap) {
             e = hipErrorInvalidValue;
             goto fail;
         }
     }
+    long_cols = NDOF * max_inc > kColCap;
     // pass 1: distinct rows per column
     ucount = reinterpret_cast<long long*>(cnt);  // reuse: N+1 int64
     TRY(hipMemsetAsync(ucount, 0, sizeof(long long) * (N + 1), s));
     {
         const unsigned g = (unsigned)(N < (1 << 20) ? N : (1 << 20));
+        const unsigned gl = (unsigned)(N < (1 << 16) ? N : (1 << 16));
         if (N > 0) {
             hipLaunchKernelGGL((csc_column_kernel<false, NDOF>), dim3(g), dim3(64), 0, s, m.t2f,
                                m.tetids,
                                b.inc_ptr, inc, N, ucount, nullptr, nullptr, nullptr, nullptr,
                                nullptr);
+            TRY(hipGetLastError());
+        }
+        if (long_cols) {
+            hipLaunchKernelGGL((csc_long_column_kernel<false, NDOF>), dim3(gl),
+                               dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
+                               ucount, nullptr, nullptr, nullptr, nullptr, nullptr);
             TRY(hipGetLastError());
         }
         TRY(hipMalloc((void**)&b.indptr, sizeof(long long) * (N + 1)));
@@ -490,6 +576,12 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
                                m.tetids,
                                b.inc_ptr, inc, N, nullptr, b.indptr, b.indices, b.perm, b.seg,
                                b.pos);
+            TRY(hipGetLastError());
+        }
+        if (long_cols) {
+            hipLaunchKernelGGL((csc_long_column_kernel<true, NDOF>), dim3(gl),
+                               dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
+                               nullptr, b.indptr, b.indices, b.perm, b.seg, b.pos);
             TRY(hipGetLastError());
         }
         TRY(hipStreamSynchronize(s));

@@ -215,15 +215,18 @@ def test_staged_gather_non_symmetric_blocks(delaunay_field, mesh, precision):
     assert torch.equal(sub.to_csc_staged(K, M, k0, work=work), sub.to_csc_combine(K, M, k0))
 
 
-def test_largest_supported_column_and_the_limit():
-    """A DOF in 102 tets (20 x 102 = 2,040 incidences: the column kernel's 2,048-key LDS
-    sort at its largest) builds, and every gather matches the oracle's CSC; a DOF in 103
-    tets is refused with EM_EUNSUPPORTED (the reference's CPU argsort has no such limit,
+@pytest.mark.parametrize("n", [102, 103, 614])
+def test_long_columns_and_the_limit(n):
+    """A DOF in n tets gives a column of 20 n COO entries. Up to 102 tets (2,040 entries)
+    the column kernel sorts them in LDS; from 103 on the long-column kernel ranks them,
+    up to 614 tets (12,280 of its 12,288 keys). Either way the pattern, perm and
+    target_csc equal the oracle's and every gather matches its CSC; a DOF in 615 tets is
+    refused with EM_EUNSUPPORTED (the reference's CPU argsort has no such limit,
     csc_cast.py:65-160)."""
     import torch
     from emerge_amd._lib import UnsupportedError
     from emerge_amd.assembly import CscMap
-    field = _fan_field(102)
+    field = _fan_field(n)
     rng = np.random.default_rng(25)
     T = field.n_tets
     er, ur = ref_tensor(T, "scalar", rng), ref_tensor(T, "diag", rng)
@@ -232,9 +235,11 @@ def test_largest_supported_column_and_the_limit():
     dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(field, er, ur, np.array([]),
                                                             local_frame=True)
     ref = oracle.CSCMap(rows, cols, field.n_field)
-    ex = cm.export("indptr", "indices")
-    assert np.array_equal(ex["indptr"].cpu().numpy(), ref.indptr)
-    assert np.array_equal(ex["indices"].cpu().numpy(), ref.indices)
+    ex = {k: v.cpu().numpy() for k, v in
+          cm.export("indptr", "indices", "perm", "target_csc").items()}
+    assert np.diff(ex["indptr"]).max() > 256
+    for k in ("indptr", "indices", "perm", "target_csc"):
+        assert np.array_equal(ex[k], getattr(ref, k)), k
     k0 = 2 * np.pi * 8e9 / 299792458.0
     K, M = plan.assemble_km(er, ur, "fp64")
     want = ref.to_csc_data(dE - k0 ** 2 * dB)
@@ -242,5 +247,6 @@ def test_largest_supported_column_and_the_limit():
     assert rel_fro(got.cpu().numpy(), want) <= 1e-12
     assert torch.equal(got, cm.to_csc_combine(K, M, k0, block_symmetric=True))
     assert torch.equal(got, cm.to_csc_combine(K, M, k0, segment_gather=True))
-    with pytest.raises(UnsupportedError, match="at most 102"):
-        CscMap(_field_plan(_fan_field(103)))
+    if n == 614:
+        with pytest.raises(UnsupportedError, match="615 tets \\(at most 614"):
+            CscMap(_field_plan(_fan_field(615)))

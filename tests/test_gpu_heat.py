@@ -128,6 +128,43 @@ def test_heat_csc_and_energy(wr90_field):
     assert rel_fro(Md, M.cpu().numpy()) <= 1e-14
 
 
+@pytest.mark.parametrize("n", [205, 1228])
+def test_heat_long_columns_and_the_limit(n):
+    """The Lagrange-2 DOFs of a fan's axis (its two vertices and its midpoint) lie in all
+    n tets: columns of 10 n COO entries, past the column kernel's 2,048-entry LDS sort
+    from 205 tets on (the long-column kernel builds them), up to 1,228 tets. Pattern and
+    values equal SciPy's COO->CSC with duplicates summed, the incidence gather equals
+    the segment gather, and 1,229 tets are refused with EM_EUNSUPPORTED."""
+    import torch
+    from emerge_amd._lib import UnsupportedError
+    from emerge_amd.heat import HeatCscMap, HeatPlan
+    from emerge_amd.mesh import Legrange2Field
+    from tests.test_gpu_csc import _fan_field
+    f = Legrange2Field(_fan_field(n).mesh)
+    T = f.n_tets
+    plan = HeatPlan.from_field(f)
+    rng = np.random.default_rng(n)
+    K, M = plan.assemble(cond=torch.tensor(rng.uniform(1, 3, T)),
+                         rho_cp=torch.tensor(rng.uniform(1, 2, T)))
+    cm = HeatCscMap(plan)
+    r, cc = plan.rowcol()
+    S = sp.coo_matrix((K.cpu().numpy(), (r.cpu().numpy(), cc.cpu().numpy())),
+                      shape=(f.n_field, f.n_field)).tocsc()
+    S.sum_duplicates()
+    S.sort_indices()
+    ex = cm.export("indptr", "indices")
+    assert np.array_equal(ex["indptr"].cpu().numpy(), S.indptr)
+    assert np.array_equal(ex["indices"].cpu().numpy(), S.indices)
+    vals = cm.to_csc(K)
+    assert rel_fro(vals.cpu().numpy(), S.data) <= 1e-14
+    assert torch.equal(vals, cm.to_csc(K, segment_gather=True))
+    assert torch.equal(cm.to_csc(K, M, 1e3), cm.to_csc(K, M, 1e3, segment_gather=True))
+    if n == 1228:
+        bigger = HeatPlan.from_field(Legrange2Field(_fan_field(1229).mesh))
+        with pytest.raises(UnsupportedError, match="1229 tets \\(at most 1228"):
+            HeatCscMap(bigger)
+
+
 def test_heat_csc_incidence_gather_equals_segment_gather(delaunay_field):
     """The column-incidence gather of real data sums every CSC entry's contributors in COO
     order, like the per-entry segment gather: the results are equal bit for bit, with and

@@ -96,6 +96,9 @@ _SIGS = {
     "em_ned2_backward": (ctypes.c_int, [P, P, ctypes.c_double, I32, P, I32, P, P]),
     "em_ned2_pec_mask": (ctypes.c_int, [P, P, I64, P, I64, P, I64, P, P]),
     "em_dof_compact": (ctypes.c_int, [I64, P, P, P, ctypes.POINTER(I64), P]),
+    "em_material_prep": (ctypes.c_int, [P, P, P, I64, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, P, P, ctypes.POINTER(I64),
+                                        ctypes.POINTER(I32), P]),
     "em_csc_select_count": (ctypes.c_int, [I64, P, P, P, I32, I32, P, I64, P,
                                            ctypes.POINTER(I64), P]),
     "em_csc_select_fill": (ctypes.c_int, [I64, P, P, P, I32, I32, P, P, P, P, P]),

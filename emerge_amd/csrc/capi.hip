@@ -778,6 +778,33 @@ int em_ned2_pec_mask(em_ned2_plan_t plan, const int64_t* conductor_tets, int64_t
     return EM_OK;
 }
 
+int em_material_prep(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                     double omega, double pec_limit, double surf_limit, void* er_out,
+                     int64_t* conductor_tets, int64_t* n_conductor,
+                     int32_t* frequency_dependent, void* stream) {
+    if (n_tets < 0 || !n_conductor || !frequency_dependent ||
+        (n_tets > 0 && (!er || !tand || !cond || !er_out || !conductor_tets))) {
+        set_error("material_prep: bad argument");
+        return EM_EINVAL;
+    }
+    if (!(omega > 0.0)) {
+        set_error("material_prep: omega must be positive");
+        return EM_EINVAL;
+    }
+    if (9 * n_tets >= (1ll << 40)) {
+        set_error("material_prep: too many tets");
+        return EM_EINVAL;
+    }
+    long long nc = 0;
+    bool fd = false;
+    EM_HIP(em::material_prep((const double2*)er, (const double2*)tand, (const double2*)cond,
+                             n_tets, omega, pec_limit, surf_limit, (double2*)er_out,
+                             (long long*)conductor_tets, nc, fd, (hipStream_t)stream));
+    *n_conductor = nc;
+    *frequency_dependent = fd ? 1 : 0;
+    return EM_OK;
+}
+
 int em_dof_compact(int64_t n, const uint8_t* keep, int64_t* new_index, int64_t* solve_ids,
                    int64_t* n_solve, void* stream) {
     if (n < 0 || !n_solve || (n > 0 && (!keep || !new_index || !solve_ids))) {

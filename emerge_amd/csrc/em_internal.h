@@ -129,6 +129,13 @@ hipError_t pec_mark(const int* t2f, long long n_tets, const long long* cond, lon
                     long long n_pec, long long n_field, unsigned char* keep, long long& bad,
                     hipStream_t s);
 // new_index[d] = position of d among the kept DOFs or -1; solve_ids = kept DOFs ascending.
+// er' = er(1 - j tand) - j cond/(omega eps0) over (3,3,T) complex128 tensors (out may be
+// er), the ascending ids of the conductor tets (cond[0,0] above either limit) and whether
+// some component lies strictly between 0 and pec_limit. Blocks until done.
+hipError_t material_prep(const double2* er, const double2* tand, const double2* cond,
+                         long long n_tets, double omega, double pec_limit, double surf_limit,
+                         double2* out, long long* conductor_tets, long long& n_conductor,
+                         bool& frequency_dependent, hipStream_t s);
 hipError_t dof_compact(long long n, const unsigned char* keep, long long* new_index,
                        long long* solve_ids, long long& n_solve, hipStream_t s);
 hipError_t csc_select_count(long long n, const long long* indptr, const long long* indices,

@@ -79,6 +79,24 @@ const char* em_last_error(void);
 const char* em_version(void);
 
 /*
+ * Material tensors of one frequency as Assembler.assemble_freq_matrix prepares them
+ * (physics/microwave/assembly/assembler.py:439-469; the same in assemble_scattering_matrix
+ * :819-868). er, tand, cond and er_out are device complex128 (3,3,T) in C order (entry
+ * (p,q) of tet t at (3p+q)T + t); er_out may be er. Computes
+ *   er_out = er * (1 - 1j*tand) - 1j*cond / (omega*EPS0)      (:453, EPS0 of const.py:23)
+ * with omega = 2*pi*f; conductor_tets (device, room for T) gets the ascending ids of the
+ * tets with cond[0,0,t] > pec_limit or > surf_limit (:461-469; the reference passes
+ * settings mw_3d_peclim = 1e8 and mw_3d_surfimplim = 1e4), *n_conductor their count, and
+ * *frequency_dependent = 1 when some component of cond lies strictly between 0 and
+ * pec_limit (:455-457). Comparisons order complex numbers as NumPy does (real part, then
+ * imaginary part). Blocks until done.
+ */
+int em_material_prep(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                     double omega, double pec_limit, double surf_limit, void* er_out,
+                     int64_t* conductor_tets, int64_t* n_conductor,
+                     int32_t* frequency_dependent, void* stream);
+
+/*
  * Create an assembly plan: copies the mesh tables to device `device`, validates
  * them (ids in range, edge/face tables equal to the sorted-tuple tables of the tets as
  * mesh3d.py:707-774 builds them, no zero-volume tet) and keeps them resident.

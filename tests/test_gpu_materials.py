@@ -1,0 +1,97 @@
+"""GPU parity of em_material_prep (materials.hip, emerge_amd.materials) against the oracle
+restatement of assembler.py:439-469, and the prepared tensors through the drop-in assembly
+with the conductor tets left out."""
+import numpy as np
+import pytest
+
+from oracle import materials as om
+from tests.conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+F = 9.3e9
+
+
+def _inputs(T, seed):
+    rng = np.random.default_rng(seed)
+    er = rng.uniform(1, 10, (3, 3, T)) + 1j * rng.normal(size=(3, 3, T))
+    tand = rng.uniform(0, 0.02, (3, 3, T)) + 1j * rng.normal(size=(3, 3, T)) * 1e-3
+    cond = rng.uniform(0, 50, (3, 3, T)) * (rng.random((3, 3, T)) < 0.3) + 0j
+    # conductors and boundary values in cond[0,0]
+    n = min(T, 40)
+    pick = rng.choice(T, n, replace=False)
+    edge = [1e4, 1e4 + 1j, 1e4 - 1j, 2e8, 1e8, 1e8 + 1j, 5e7, 1j, -1j, 3e6]
+    cond[0, 0, pick] = np.resize(edge, n)
+    return er, tand, cond
+
+
+def _close(got, ref, er, tand, cond):
+    """Within a few roundings of the terms' size: the GPU follows NumPy's fused
+    complex products; a NumPy build without them differs by that much, at most."""
+    scale = np.abs(er) * (1 + np.abs(tand)) + np.abs(cond) / (2 * np.pi * F * om.EPS0)
+    return bool(np.all(np.abs(got - ref) <= 4 * 2.0 ** -52 * scale))
+
+
+@pytest.mark.parametrize("T", [0, 1, 7, 20000])
+def test_material_prep_matches_oracle(T):
+    import torch
+    from emerge_amd.materials import prepare_materials
+    er, tand, cond = _inputs(max(T, 1), 5)
+    er, tand, cond = er[:, :, :T], tand[:, :, :T], cond[:, :, :T]
+    ref, ids_ref, fd_ref = om.prepare_materials(er, tand, cond, F)
+    got, ids, fd = prepare_materials(er, tand, cond, F)
+    assert got.shape == (3, 3, T) and got.is_cuda
+    assert _close(got.cpu().numpy(), ref, er, tand, cond)
+    assert ids.dtype == np.int64 and np.array_equal(ids, ids_ref)
+    assert fd == fd_ref
+    # in place, on device tensors; PEC-only conductivities are not frequency dependent
+    e = torch.as_tensor(np.ascontiguousarray(er), device="cuda")
+    c2 = np.where(np.abs(cond) > 0, 2e9 + 0j, 0j)
+    ref2, ids2, fd2 = om.prepare_materials(er, tand, c2, F)
+    out, ids_b, fd_b = prepare_materials(e, tand, c2, F, out=e)
+    assert out.data_ptr() == e.data_ptr()
+    assert _close(e.cpu().numpy(), ref2, er, tand, c2)
+    assert np.array_equal(ids_b, ids2) and fd_b == fd2 and not fd_b
+
+
+def test_bad_arguments():
+    import torch
+    from emerge_amd.materials import prepare_materials
+    er, tand, cond = _inputs(10, 6)
+    with pytest.raises(ValueError):
+        prepare_materials(er, tand[:, :, :9], cond, F)
+    with pytest.raises(ValueError):
+        prepare_materials(er, tand, cond, F,
+                          out=torch.empty((3, 3, 10), dtype=torch.complex64, device="cuda"))
+
+
+def test_prepared_materials_through_the_drop_in(wr90_field):
+    """assemble_freq_matrix's flow (assembler.py:439-494): prepared er, conductor tets
+    removed, then tet_mass_stiffness_matrices and to_csc(E - K0^2 B); equal to the oracle
+    on the same prepared tensors and conductor list."""
+    import oracle
+    from emerge_amd.curlcurl import tet_mass_stiffness_matrices
+    from emerge_amd.materials import prepare_materials
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(7)
+    er = np.zeros((3, 3, T), complex)
+    ur = np.zeros((3, 3, T), complex)
+    tand = np.zeros((3, 3, T), complex)
+    cond = np.zeros((3, 3, T), complex)
+    for d in range(3):
+        er[d, d] = rng.uniform(1, 4, T)
+        ur[d, d] = 1.0
+        tand[d, d] = rng.uniform(0, 0.01, T)
+        cond[d, d] = np.where(np.arange(T) % 97 == 0, 5.8e7, 0.0)  # copper tets
+    ref, ids_ref, fd_ref = om.prepare_materials(er, tand, cond, F)
+    got, ids, fd = prepare_materials(er, tand, cond, F)
+    assert np.array_equal(ids, ids_ref) and ids.size == len(range(0, T, 97)) and fd and fd_ref
+    E, B, cmap = tet_mass_stiffness_matrices(f, got.cpu().numpy(), ur, ids)
+    assert E.shape == (400 * (T - ids.size),)
+    k0 = 2 * np.pi * F / 299792458.0
+    A = cmap.to_csc(E - B * k0 ** 2)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids)
+    A_ref = oracle.CSCMap(rows, cols, f.n_field).to_csc(dE - dB * k0 ** 2)
+    assert np.array_equal(A.indptr, A_ref.indptr) and np.array_equal(A.indices, A_ref.indices)
+    assert rel_fro(A.data, A_ref.data) <= 1e-12

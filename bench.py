@@ -430,6 +430,47 @@ def suite_surface_interp(args) -> dict:
                     "counts; interp oracle uses OpenMP threads"}
 
 
+def suite_materials(args) -> dict:
+    """Material preparation of assemble_freq_matrix (assembler.py:439-469): er(1 - j tand)
+    - j cond/(w eps0), the conductor ids and the frequency-dependence flag, on (3,3,T)
+    complex128 tensors with T = the bench mesh's tet count and 1% copper tets. One call =
+    the kernel, the id compaction and the host copy of the id count (the call blocks); the
+    bandwidth counts 576 B per tet (3 x 144 B read, 144 B written). CPU line: the oracle
+    (the NumPy expression plus the reference's per-tet loop) on 100k tets, scaled."""
+    import numpy as np
+    import torch
+    from oracle import materials as om
+    from emerge_amd.materials import prepare_materials
+    T = 2_815_239
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 11)
+
+    def rand(scale, shift=0.0):
+        re = torch.rand((3, 3, T), generator=g, device="cuda", dtype=torch.float64)
+        im = torch.rand((3, 3, T), generator=g, device="cuda", dtype=torch.float64)
+        return torch.complex(re * scale + shift, im * scale * 1e-3)
+
+    er, tand = rand(9.0, 1.0), rand(0.02)
+    cond = torch.zeros((3, 3, T), dtype=torch.complex128, device="cuda")
+    cu = torch.arange(0, T, 100, device="cuda")
+    for d in range(3):
+        cond[d, d, cu] = 5.8e7
+    out = torch.empty_like(er)
+    f = 10e9
+    _, ids, fd = prepare_materials(er, tand, cond, f, out=out)
+    ms, _ = time_launches(lambda: prepare_materials(er, tand, cond, f, out=out), 10, 2)
+    n = 100_000
+    e_np, t_np, c_np = (x[:, :, :n].cpu().numpy() for x in (er, tand, cond))
+    t0 = time.perf_counter()
+    _, ids_cpu, _ = om.prepare_materials(e_np, t_np, c_np, f)
+    cpu_ms = (time.perf_counter() - t0) * 1e3 * T / n
+    return {"tets": T, "conductor_tets": int(ids.size), "frequency_dependent": fd,
+            "ms_per_call": ms, "gb_per_s_call": T * 576 / (ms * 1e-3) / 1e9,
+            "ids_equal_oracle_on_sample": bool(np.array_equal(ids[ids < n], ids_cpu)),
+            "cpu_ms_oracle_1thread_scaled": cpu_ms,
+            "note": "call time includes the id compaction's scan, its temporaries and the "
+                    "blocking copy of the count"}
+
+
 def suite_heat(args) -> dict:
     """Lagrange-2 heat assembly (conductance and capacity, FP64 COO) on a ~2.8M-tet mesh
     (the bench mesh), its CSC map and the per-step to_csc(K + M / dt); CPU line: the
@@ -549,7 +590,7 @@ def main() -> int:
                          "32-mesh batch and the backward on 1M tets")
     ap.add_argument("--only", default="",
                     help="with --suite: comma list among sweep,batch,backward,csc_pec,"
-                         "surface_interp,heat")
+                         "surface_interp,heat,materials")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -687,7 +728,8 @@ def main() -> int:
                     ("backward", "backward_1M", suite_backward),
                     ("csc_pec", "csc_pec_500k", suite_csc_pec),
                     ("surface_interp", "surface_interp_500k", suite_surface_interp),
-                    ("heat", "heat_2.8M", suite_heat))
+                    ("heat", "heat_2.8M", suite_heat),
+                    ("materials", "materials_2.8M", suite_materials))
             only = set(args.only.split(",")) if args.only else None
             for short, key, fn in runs:
                 if only is None or short in only:

@@ -38,7 +38,7 @@ __device__ __forceinline__ bool clt(double2 a, double b) {  // a < b + 0j
 __global__ __launch_bounds__(256) void material_prep_kernel(
     const double2* er, const double2* __restrict__ tand, const double2* __restrict__ cond,
     long long n_tets, double inv_d, double pec_limit, double surf_limit, double2* out,
-    unsigned char* __restrict__ conductor, unsigned int* __restrict__ freq_dep) {
+    long long* __restrict__ conductor, unsigned int* __restrict__ freq_dep) {
 #pragma clang fp contract(off)
     const long long n = 9 * n_tets;
     bool fd = false;
@@ -53,9 +53,22 @@ __global__ __launch_bounds__(256) void material_prep_kernel(
         const double qr = (-c.y) * inv_d, qi = c.x * inv_d;
         out[k] = make_double2(pr - qr, pi - qi);
         if (k < n_tets) conductor[k] = (cgt(c, pec_limit) || cgt(c, surf_limit)) ? 1 : 0;
+        if (k == 0) conductor[n_tets] = 0;  // the scan's total lands here
         fd = fd || (cgt(c, 0.0) && clt(c, pec_limit));
     }
     if (__ballot(fd) && (threadIdx.x & 63) == 0) atomicOr(freq_dep, 1u);
+}
+
+// scan = exclusive scan of the conductor flags, [T + 1]: tet t is a conductor when its
+// entry steps up, and goes to position scan[t] (ascending ids).
+__global__ __launch_bounds__(256) void conductor_ids_kernel(const long long* __restrict__ scan,
+                                                            long long n_tets,
+                                                            long long* __restrict__ ids) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n_tets;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long p = scan[t];
+        if (scan[t + 1] != p) ids[p] = t;
+    }
 }
 
 }  // namespace
@@ -64,34 +77,38 @@ hipError_t material_prep(const double2* er, const double2* tand, const double2* 
                          long long n_tets, double omega, double pec_limit, double surf_limit,
                          double2* out, long long* conductor_tets, long long& n_conductor,
                          bool& frequency_dependent, hipStream_t s) {
-    unsigned char* flags = nullptr;
-    long long* new_index = nullptr;
+    long long* scan = nullptr;  // conductor flags, then their exclusive scan
     unsigned int* dfd = nullptr;
     unsigned int hfd = 0;
-    const size_t nt = (size_t)(n_tets > 0 ? n_tets : 1);
+    long long nc = 0;
     hipError_t e;
-    if ((e = hipMalloc((void**)&flags, nt)) != hipSuccess) goto done;
-    if ((e = hipMalloc((void**)&new_index, sizeof(long long) * nt)) != hipSuccess) goto done;
+    n_conductor = 0;
+    frequency_dependent = false;
+    if (n_tets == 0) return hipSuccess;
+    if ((e = hipMalloc((void**)&scan, sizeof(long long) * (n_tets + 1))) != hipSuccess) goto done;
     if ((e = hipMalloc((void**)&dfd, sizeof(unsigned int))) != hipSuccess) goto done;
     if ((e = hipMemsetAsync(dfd, 0, sizeof(unsigned int), s)) != hipSuccess) goto done;
-    if (n_tets > 0) {
+    {
         const double inv_d = 1.0 / (omega * kEps0);
         hipLaunchKernelGGL(material_prep_kernel, dim3(grid_for(9 * n_tets)), dim3(256), 0, s, er,
-                           tand, cond, n_tets, inv_d, pec_limit, surf_limit, out, flags, dfd);
+                           tand, cond, n_tets, inv_d, pec_limit, surf_limit, out, scan, dfd);
         if ((e = hipGetLastError()) != hipSuccess) goto done;
     }
-    // ascending ids of the flagged tets (blocks until done)
-    if ((e = dof_compact(n_tets, flags, new_index, conductor_tets, n_conductor, s)) !=
+    if ((e = exclusive_scan_i64(scan, scan, n_tets + 1, s)) != hipSuccess) goto done;
+    hipLaunchKernelGGL(conductor_ids_kernel, dim3(grid_for(n_tets)), dim3(256), 0, s, scan, n_tets,
+                       conductor_tets);
+    if ((e = hipGetLastError()) != hipSuccess) goto done;
+    if ((e = hipMemcpyAsync(&nc, scan + n_tets, sizeof(nc), hipMemcpyDeviceToHost, s)) !=
         hipSuccess)
         goto done;
     if ((e = hipMemcpyAsync(&hfd, dfd, sizeof(hfd), hipMemcpyDeviceToHost, s)) != hipSuccess)
         goto done;
-    e = hipStreamSynchronize(s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) goto done;
+    n_conductor = nc;
     frequency_dependent = hfd != 0;
 done:
     (void)hipStreamSynchronize(s);
-    (void)hipFree(flags);
-    (void)hipFree(new_index);
+    (void)hipFree(scan);
     (void)hipFree(dfd);
     return e;
 }

@@ -1,6 +1,7 @@
 // Material tensors of one frequency, as Assembler.assemble_freq_matrix prepares them before
 // the volume assembly (src/emerge/_emerge/physics/microwave/assembly/assembler.py:439-469;
-// the same lines in assemble_scattering_matrix, :819-868):
+// the same lines in assemble_scattering_matrix, :819-868, and assemble_eig_matrix,
+// :1119-1142):
 //   er' = er * (1 - 1j*tand) - 1j*cond / (W0*EPS0)      every one of the 9 components
 //   conductor tets: cond[0,0,t] > mw_3d_peclim or cond[0,0,t] > mw_3d_surfimplim
 //   frequency dependent: some component has 0 < cond < mw_3d_peclim

@@ -1,8 +1,9 @@
 """Material tensors of one frequency on the device (``em_material_prep``).
 
 Mirrors the preparation in ``Assembler.assemble_freq_matrix``
-(src/emerge/_emerge/physics/microwave/assembly/assembler.py:439-469, and
-``assemble_scattering_matrix`` :819-868) that precedes ``tet_mass_stiffness_matrices``:
+(src/emerge/_emerge/physics/microwave/assembly/assembler.py:439-469; the same in
+``assemble_scattering_matrix`` :819-868 and ``assemble_eig_matrix`` :1119-1142) that
+precedes ``tet_mass_stiffness_matrices``:
 
 * ``er = er * (1 - 1j*tand) - 1j*cond / (W0*EPS0)`` with ``W0 = 2*pi*f`` (:423, :453);
 * ``is_frequency_dependent``: some conductivity component with ``0 < cond < mw_3d_peclim``
@@ -52,8 +53,8 @@ def prepare_materials(er, tand, cond, frequency: float, peclim: float = PECLIM,
     c = _dev_tensor(cond, device, T)
     if out is None:
         out = torch.empty_like(e)
-    elif out.shape != e.shape or out.dtype != torch.complex128 or not out.is_contiguous() \
-            or out.device != e.device:
+    elif not isinstance(out, torch.Tensor) or out.shape != e.shape \
+            or out.dtype != torch.complex128 or not out.is_contiguous() or out.device != e.device:
         raise ValueError("out must be a contiguous complex128 (3, 3, T) tensor on the device")
     ids = torch.empty(max(T, 1), dtype=torch.int64, device=device)
     n = ctypes.c_int64()

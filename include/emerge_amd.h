@@ -81,7 +81,7 @@ const char* em_version(void);
 /*
  * Material tensors of one frequency as Assembler.assemble_freq_matrix prepares them
  * (physics/microwave/assembly/assembler.py:439-469; the same in assemble_scattering_matrix
- * :819-868). er, tand, cond and er_out are device complex128 (3,3,T) in C order (entry
+ * :819-868 and assemble_eig_matrix :1119-1142). er, tand, cond and er_out are device complex128 (3,3,T) in C order (entry
  * (p,q) of tet t at (3p+q)T + t); er_out may be er. Computes
  *   er_out = er * (1 - 1j*tand) - 1j*cond / (omega*EPS0)      (:453, EPS0 of const.py:23)
  * with omega = 2*pi*f; conductor_tets (device, room for T) gets the ascending ids of the

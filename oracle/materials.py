@@ -2,7 +2,8 @@
 
 CPU restatement of the material preparation in ``Assembler.assemble_freq_matrix``
 (src/emerge/_emerge/physics/microwave/assembly/assembler.py:439-469; the same lines in
-``assemble_scattering_matrix``, :819-868), which runs before the volume assembly:
+``assemble_scattering_matrix``, :819-868, and ``assemble_eig_matrix``, :1119-1142), which
+runs before the volume assembly:
 
 * ``er' = er * (1 - 1j*tand) - 1j*cond / (W0*EPS0)`` on the (3,3,T) complex128 tensors
   (:453), with ``W0 = 2*pi*f`` (:423) and ``EPS0`` of const.py:23, evaluated as NumPy

@@ -7,9 +7,10 @@
 //   1. incidence lists: for every DOF d, the (ia, jj) with t2f[ia][jj] == d
 //      (histogram + exclusive scan + fill, then each list sorted so the build is
 //      deterministic);
-//   2. column c's COO entries are the 20 rows of each incidence; one wavefront ranks
+//   2. column c's COO entries are the 20 rows of each incidence; one wavefront sorts
 //      its (row, position) keys in LDS, flags the first entry of every distinct row and
-//      scans the flags with ballots. Pass 1 counts distinct rows (-> indptr by scan),
+//      scans the flags with ballots (columns of more than 2,048 keys: one workgroup ranks
+//      them, csc_long_column_kernel). Pass 1 counts distinct rows (-> indptr by scan),
 //      pass 2 writes indices, perm (COO index of every entry in (col, row, k) order,
 //      the reference's fully_sorted_perm) and seg (first perm position of every CSC
 //      entry; the reference's target_csc is its inverse).
@@ -484,8 +485,8 @@ static unsigned grid_for(long long n, long long cap = 65536 * 4) {
 }
 
 // Builds the pattern of a DOF map t2f [T][NDOF] over n_assy assembly indices (N DOFs).
-// On success the caller owns b's buffers. The incidence-gather row positions (pos) are
-// built for NDOF = 20 only.
+// On success the caller owns b's buffers, including the incidence lists and the
+// incidence-gather row positions (pos).
 template <int NDOF>
 static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_assy, long long N,
                               CscBuild& b, long long& max_inc, hipStream_t s) {
@@ -494,7 +495,6 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
         const int* tetids;
         long long n_assy;
     } m = {t2f, tetids, n_assy};
-    constexpr bool kPos = true;
     const long long nnz_coo = m.n_assy * NDOF * NDOF;
     hipError_t e;
     unsigned long long *cnt = nullptr, *maxlen = nullptr;
@@ -511,9 +511,7 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
     TRY(hipMalloc((void**)&maxlen, sizeof(unsigned long long)));
     TRY(hipMalloc((void**)&inc,
                   sizeof(int) * (size_t)(m.n_assy * NDOF > 0 ? m.n_assy * NDOF : 1)));
-    if (kPos)
-        TRY(hipMalloc((void**)&b.pos, sizeof(unsigned short) *
-                                          (size_t)(nnz_coo > 0 ? nnz_coo : 1)));
+    TRY(hipMalloc((void**)&b.pos, sizeof(unsigned short) * (size_t)(nnz_coo > 0 ? nnz_coo : 1)));
     TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * (N + 1), s));
     TRY(hipMemsetAsync(maxlen, 0, sizeof(unsigned long long), s));
     if (m.n_assy > 0) {

@@ -89,7 +89,8 @@ const char* em_version(void);
  * settings mw_3d_peclim = 1e8 and mw_3d_surfimplim = 1e4), *n_conductor their count, and
  * *frequency_dependent = 1 when some component of cond lies strictly between 0 and
  * pec_limit (:455-457). Comparisons order complex numbers as NumPy does (real part, then
- * imaginary part). Blocks until done.
+ * imaginary part). omega must be positive (EM_EINVAL otherwise; the reference divides by
+ * zero at f = 0). Blocks until done.
  */
 int em_material_prep(const void* er, const void* tand, const void* cond, int64_t n_tets,
                      double omega, double pec_limit, double surf_limit, void* er_out,

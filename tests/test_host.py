@@ -82,6 +82,28 @@ def test_plan_create_rejects_bad_arguments_without_gpu():
     assert L.em_ned2_plan_destroy(None) == _lib.EM_OK
 
 
+def test_material_prep_rejects_bad_arguments_without_gpu():
+    """em_material_prep checks its arguments before any device call: a negative tet
+    count, missing pointers and a non-positive omega (the reference divides by zero at
+    f = 0) give EM_EINVAL."""
+    L = _lib.lib()
+    n = ctypes.c_int64(-5)
+    fd = ctypes.c_int32(-5)
+    fake = ctypes.c_void_p(4096)  # never dereferenced: refused first
+    st = L.em_material_prep(fake, fake, fake, 10, 0.0, 1e8, 1e4, fake, fake, ctypes.byref(n),
+                            ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL and b"omega" in L.em_last_error()
+    st = L.em_material_prep(fake, fake, fake, -1, 1.0, 1e8, 1e4, fake, fake, ctypes.byref(n),
+                            ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL
+    st = L.em_material_prep(fake, None, fake, 10, 1.0, 1e8, 1e4, fake, fake, ctypes.byref(n),
+                            ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL
+    st = L.em_material_prep(fake, fake, fake, 10, 1.0, 1e8, 1e4, fake, fake, None,
+                            ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL and n.value == -5 and fd.value == -5
+
+
 def test_status_to_exception():
     with pytest.raises(_lib.UnsupportedError):
         _lib.check(_lib.EM_EUNSUPPORTED)

@@ -1,0 +1,111 @@
+"""Device-resident volume part of ``Assembler.assemble_freq_matrix``.
+
+Follows src/emerge/_emerge/physics/microwave/assembly/assembler.py:423-494:
+
+1. ``W0 = 2*pi*f``, ``K0 = W0 / C0`` (:423-424, const.py:20);
+2. the material preparation ``er(1 - j tand) - j cond/(W0 EPS0)``, the frequency
+   dependence and the conductor tets (:439-469) -- ``em_material_prep``;
+3. the element matrices without the conductor tets (:479-485), cached as the reference's
+   ``cached_matrices`` and reused while nothing is frequency dependent (:471-478);
+4. ``cached_cscmap.to_csc(Evec - Bvec * K0**2)`` (:494) -- the fused CSC combine.
+
+Everything stays on the GPU; the result is the CSC value array of ``A`` with the map's
+``indptr``/``indices``. The cached matrices and CSC map belong to the conductor set they
+were built for (the reference reuses its cached map whatever the conductor set; here a
+new set gets its own plan and map). The boundary-condition steps that follow in the
+reference (PEC, Robin, ports) are in ``emerge_amd.pec``, ``emerge_amd.robin`` and
+``emerge_amd.ports``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .assembly import CscMap, Material
+from .curlcurl import plan_for
+from .materials import PECLIM, SURFIMPLIM, prepare_materials
+
+C0 = 299792458  # const.py:20
+
+_OFF = ([0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1])
+
+
+def _prepared_kind(er, tand, cond) -> int:
+    """Structure of er(1 - j tand) - j cond/(w eps0) known from host inputs: diagonal when
+    er and cond are (an off-diagonal er of 0 stays 0 whatever tand is), scalar when every
+    input's three diagonal entries are equal. Device inputs are taken as full."""
+    if any(isinstance(x, torch.Tensor) for x in (er, tand, cond)):
+        return _lib.EM_MAT_FULL
+    er, tand, cond = (np.asarray(x) for x in (er, tand, cond))
+    if np.any(er[_OFF]) or np.any(cond[_OFF]):
+        return _lib.EM_MAT_FULL
+    if all(np.array_equal(a[0, 0], a[1, 1]) and np.array_equal(a[0, 0], a[2, 2])
+           for a in (er, tand, cond)):
+        return _lib.EM_MAT_SCALAR
+    return _lib.EM_MAT_DIAG
+
+
+@dataclass
+class FreqMatrix:
+    """``A = K - K0^2 M`` in CSC form on the device."""
+    data: torch.Tensor          # CSC values, complex128 (fp64) or complex64 (mixed)
+    cscmap: CscMap              # indptr / indices via cscmap.export
+    conductor_tets: np.ndarray  # int64, ascending
+    is_frequency_dependent: bool
+    k0: float
+
+    def to_scipy(self):
+        from .pec import to_scipy
+        ex = self.cscmap.export("indptr", "indices")
+        return to_scipy(ex["indptr"], ex["indices"], self.data, self.cscmap.N)
+
+
+class VolumeAssembler:
+    """Holds the reference assembler's caches for one field: the element matrices of the
+    last conductor set (``cached_matrices``) and each plan's CSC map (``cached_cscmap``).
+    ``assemblies`` counts the element-matrix launches (reuse leaves it unchanged)."""
+
+    def __init__(self, field, precision: str = "fp64", peclim: float = PECLIM,
+                 surfimplim: float = SURFIMPLIM):
+        self.field = field
+        self.precision = precision
+        self.peclim = peclim
+        self.surfimplim = surfimplim
+        self.cached_matrices = None  # (plan, K, M)
+        self._map = None  # (plan, CscMap)
+        self.assemblies = 0
+
+    def assemble_freq_matrix(self, er, ur, tand, cond, frequency: float,
+                             cache_matrices: bool = True,
+                             frequency_dependent: bool = False) -> FreqMatrix:
+        """``er``, ``ur``, ``tand``, ``cond``: complex (3, 3, T) tensors as the reference
+        fills them from its materials (:440-450); ``frequency_dependent`` stands for the
+        materials' own ``frequency_dependent`` flags (:433-436)."""
+        W0 = 2 * np.pi * frequency
+        K0 = W0 / C0
+        er_d, conductor_tets, fd = prepare_materials(er, tand, cond, frequency, self.peclim,
+                                                     self.surfimplim)
+        fd = fd or frequency_dependent
+        plan = plan_for(self.field, conductor_tets)
+        T = plan.n_tets
+        kind = _prepared_kind(er, tand, cond)
+        comp = {_lib.EM_MAT_FULL: T, _lib.EM_MAT_DIAG: 4 * T, _lib.EM_MAT_SCALAR: 0}[kind]
+        er_m = Material(er_d, kind, 1, comp)
+        ur_m = plan.material(ur)
+        c = self.cached_matrices
+        if cache_matrices and not fd and c is not None and c[0] is plan:
+            K, M = c[1], c[2]
+        else:
+            K, M = plan.assemble_km(er_m, ur_m, self.precision)
+            self.assemblies += 1
+            self.cached_matrices = (plan, K, M)
+        if self._map is None or self._map[0] is not plan:
+            self._map = (plan, CscMap(plan))
+        cm = self._map[1]
+        # scalar or diagonal tensors give symmetric element blocks: mirrored reads
+        sym = kind != _lib.EM_MAT_FULL and ur_m.kind != _lib.EM_MAT_FULL
+        data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
+        return FreqMatrix(data, cm, conductor_tets, fd, K0)

@@ -95,3 +95,52 @@ def test_prepared_materials_through_the_drop_in(wr90_field):
     A_ref = oracle.CSCMap(rows, cols, f.n_field).to_csc(dE - dB * k0 ** 2)
     assert np.array_equal(A.indptr, A_ref.indptr) and np.array_equal(A.indices, A_ref.indices)
     assert rel_fro(A.data, A_ref.data) <= 1e-12
+
+
+def test_volume_assembler_flow_and_caching(wr90_field):
+    """emerge_amd.assembler.VolumeAssembler follows assemble_freq_matrix (assembler.py:
+    423-494): A equals the oracle's (prepared materials, conductor tets removed, CSC of
+    E - K0^2 B) at each frequency. The element matrices are reused while nothing is
+    frequency dependent (a lossy dielectric, or PEC-only conductors) and rebuilt when a
+    finite conductivity is present or caching is off."""
+    import oracle
+    from emerge_amd.assembler import VolumeAssembler
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(8)
+    er = np.zeros((3, 3, T), complex)
+    ur = np.zeros((3, 3, T), complex)
+    tand = np.zeros((3, 3, T), complex)
+    eps, td = rng.uniform(1, 4, T), rng.uniform(0, 0.01, T)
+    for d in range(3):
+        er[d, d], tand[d, d], ur[d, d] = eps, td, 1.0
+    copper, pec = np.zeros_like(er), np.zeros_like(er)
+    for d in range(3):
+        copper[d, d] = np.where(np.arange(T) % 97 == 0, 5.8e7, 0.0)
+        pec[d, d] = np.where(np.arange(T) % 89 == 0, 1e9, 0.0)
+
+    def oracle_A(cond, freq):
+        ref, ids, fd = om.prepare_materials(er, tand, cond, freq)
+        dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids)
+        k0 = 2 * np.pi * freq / 299792458
+        return oracle.CSCMap(rows, cols, f.n_field).to_csc(dE - dB * k0 ** 2), ids, fd
+
+    for cond, dep in ((np.zeros_like(er), False), (pec, False), (copper, True)):
+        va = VolumeAssembler(f)
+        for i, freq in enumerate((8.2e9, 9.7e9, 11.1e9)):
+            r = va.assemble_freq_matrix(er, ur, tand, cond, freq)
+            A_ref, ids, fd = oracle_A(cond, freq)
+            assert r.is_frequency_dependent == fd == dep
+            assert np.array_equal(r.conductor_tets, ids)
+            A = r.to_scipy()
+            assert np.array_equal(A.indptr, A_ref.indptr)
+            assert np.array_equal(A.indices, A_ref.indices)
+            assert rel_fro(A.data, A_ref.data) <= 1e-12, (dep, freq)
+            assert va.assemblies == (i + 1 if dep else 1)
+        va.assemble_freq_matrix(er, ur, tand, cond, 10e9, cache_matrices=False)
+        assert va.assemblies == (4 if dep else 2)
+    # mixed precision through the same flow
+    a64 = VolumeAssembler(f).assemble_freq_matrix(er, ur, tand, copper, 10e9).data
+    a32 = VolumeAssembler(f, "mixed").assemble_freq_matrix(er, ur, tand, copper, 10e9).data
+    assert a32.dtype.itemsize == 8
+    assert rel_fro(a32.cpu().numpy().astype(complex), a64.cpu().numpy()) < 1e-6

@@ -213,3 +213,48 @@ def test_mesh_errors():
         TetMesh(np.zeros((3, 4)), np.array([[0], [1], [2], [2]]))
     with pytest.raises(MeshError):
         TetMesh(np.zeros((3, 4)), np.array([[0], [1], [2], [4]]))
+
+
+def test_prepared_material_structure_matches_the_arithmetic():
+    """The structure the volume assembler assigns to er(1 - j tand) - j cond/(w eps0)
+    from its host inputs is the structure of the oracle's result: scalar only when every
+    component on the diagonal is equal, diagonal when no off-diagonal entry is nonzero
+    (an off-diagonal loss tangent alone keeps it diagonal), full otherwise."""
+    from emerge_amd.assembler import _prepared_kind
+    from oracle import materials as om
+    rng = np.random.default_rng(4)
+    T = 50
+    off = ([0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1])
+
+    def diag(vals):
+        a = np.zeros((3, 3, T), complex)
+        for d in range(3):
+            a[d, d] = vals[d]
+        return a
+
+    s = rng.uniform(1, 5, T) + 0j
+    cases = {
+        "scalar": (diag([s, s, s]), diag([s / 100] * 3), diag([s * 0] * 3)),
+        "diag_er": (diag([s, s * 2, s]), diag([s / 100] * 3), diag([s * 0] * 3)),
+        "diag_cond": (diag([s, s, s]), diag([s / 100] * 3), diag([s, s * 0, s])),
+        "offdiag_tand": (diag([s, s, s]), diag([s / 100] * 3) + 0.01 * (1 - np.eye(3))[:, :, None],
+                         diag([s * 0] * 3)),
+        "full_er": (diag([s, s, s]) + 0.1 * (1 - np.eye(3))[:, :, None], diag([s / 100] * 3),
+                    diag([s * 0] * 3)),
+        "full_cond": (diag([s, s, s]), diag([s * 0] * 3),
+                      np.where(np.eye(3)[:, :, None] > 0, 0j, 3.0 + 0j) * np.ones((3, 3, T))),
+    }
+    want = {"scalar": _lib.EM_MAT_SCALAR, "diag_er": _lib.EM_MAT_DIAG,
+            "diag_cond": _lib.EM_MAT_DIAG, "offdiag_tand": _lib.EM_MAT_SCALAR,
+            "full_er": _lib.EM_MAT_FULL, "full_cond": _lib.EM_MAT_FULL}
+    for name, (er, tand, cond) in cases.items():
+        kind = _prepared_kind(er, tand, cond)
+        assert kind == want[name], name
+        out, _, _ = om.prepare_materials(er, tand, cond, 10e9)
+        if kind != _lib.EM_MAT_FULL:
+            assert not np.any(out[off]), name
+        if kind == _lib.EM_MAT_SCALAR:
+            assert np.array_equal(out[0, 0], out[1, 1]) and np.array_equal(out[0, 0], out[2, 2])
+        if kind == _lib.EM_MAT_DIAG:
+            assert not (np.array_equal(out[0, 0], out[1, 1]) and
+                        np.array_equal(out[0, 0], out[2, 2])), name

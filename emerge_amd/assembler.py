@@ -1,6 +1,8 @@
 """Device-resident volume part of ``Assembler.assemble_freq_matrix``.
 
-Follows src/emerge/_emerge/physics/microwave/assembly/assembler.py:423-494:
+Follows src/emerge/_emerge/physics/microwave/assembly/assembler.py:423-494 (the same
+steps in ``assemble_scattering_matrix``, :819-868; ``assemble_eig_matrix``, :1115-1153,
+converts K and M separately instead of combining them):
 
 1. ``W0 = 2*pi*f``, ``K0 = W0 / C0`` (:423-424, const.py:20);
 2. the material preparation ``er(1 - j tand) - j cond/(W0 EPS0)``, the frequency
@@ -48,6 +50,11 @@ def _prepared_kind(er, tand, cond) -> int:
     return _lib.EM_MAT_DIAG
 
 
+def _symmetric(er_m: Material, ur_m: Material) -> bool:
+    """Scalar or diagonal tensors give symmetric element blocks: mirrored CSC reads."""
+    return er_m.kind != _lib.EM_MAT_FULL and ur_m.kind != _lib.EM_MAT_FULL
+
+
 @dataclass
 class FreqMatrix:
     """``A = K - K0^2 M`` in CSC form on the device."""
@@ -86,15 +93,8 @@ class VolumeAssembler:
         materials' own ``frequency_dependent`` flags (:433-436)."""
         W0 = 2 * np.pi * frequency
         K0 = W0 / C0
-        er_d, conductor_tets, fd = prepare_materials(er, tand, cond, frequency, self.peclim,
-                                                     self.surfimplim)
+        plan, er_m, ur_m, conductor_tets, fd = self._prepare(er, ur, tand, cond, frequency)
         fd = fd or frequency_dependent
-        plan = plan_for(self.field, conductor_tets)
-        T = plan.n_tets
-        kind = _prepared_kind(er, tand, cond)
-        comp = {_lib.EM_MAT_FULL: T, _lib.EM_MAT_DIAG: 4 * T, _lib.EM_MAT_SCALAR: 0}[kind]
-        er_m = Material(er_d, kind, 1, comp)
-        ur_m = plan.material(ur)
         c = self.cached_matrices
         if cache_matrices and not fd and c is not None and c[0] is plan:
             K, M = c[1], c[2]
@@ -102,10 +102,37 @@ class VolumeAssembler:
             K, M = plan.assemble_km(er_m, ur_m, self.precision)
             self.assemblies += 1
             self.cached_matrices = (plan, K, M)
+        cm = self._cscmap(plan)
+        data = cm.to_csc_combine(K, M, K0, block_symmetric=_symmetric(er_m, ur_m))
+        return FreqMatrix(data, cm, conductor_tets, fd, K0)
+
+    def assemble_eig_matrices(self, er, ur, tand, cond, frequency: float):
+        """The volume step of ``assemble_eig_matrix`` (assembler.py:1115-1153): the same
+        material preparation at the compilation frequency, the element matrices without the
+        conductor tets, then ``to_csc`` of the stiffness and of the mass separately
+        (:1151-1152). Returns ``(K, M, cscmap, conductor_tets)`` with ``K`` and ``M`` CSC
+        value arrays on the device; no caching (the reference rebuilds them each call)."""
+        plan, er_m, ur_m, conductor_tets, _ = self._prepare(er, ur, tand, cond, frequency)
+        K, M = plan.assemble_km(er_m, ur_m, self.precision)
+        self.assemblies += 1
+        cm = self._cscmap(plan)
+        sym = _symmetric(er_m, ur_m)
+        return (cm.to_csc(K, block_symmetric=sym), cm.to_csc(M, block_symmetric=sym), cm,
+                conductor_tets)
+
+    def _prepare(self, er, ur, tand, cond, frequency):
+        """Prepared materials on the device, the plan of their conductor set and the
+        kernels' descriptors of er' and ur."""
+        er_d, conductor_tets, fd = prepare_materials(er, tand, cond, frequency, self.peclim,
+                                                     self.surfimplim)
+        plan = plan_for(self.field, conductor_tets)
+        T = plan.n_tets
+        kind = _prepared_kind(er, tand, cond)
+        comp = {_lib.EM_MAT_FULL: T, _lib.EM_MAT_DIAG: 4 * T, _lib.EM_MAT_SCALAR: 0}[kind]
+        return plan, Material(er_d, kind, 1, comp), plan.material(ur), conductor_tets, fd
+
+    def _cscmap(self, plan) -> CscMap:
+        """The CSC map of ``plan`` (the reference's ``cached_cscmap``)."""
         if self._map is None or self._map[0] is not plan:
             self._map = (plan, CscMap(plan))
-        cm = self._map[1]
-        # scalar or diagonal tensors give symmetric element blocks: mirrored reads
-        sym = kind != _lib.EM_MAT_FULL and ur_m.kind != _lib.EM_MAT_FULL
-        data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
-        return FreqMatrix(data, cm, conductor_tets, fd, K0)
+        return self._map[1]

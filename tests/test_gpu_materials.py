@@ -144,3 +144,32 @@ def test_volume_assembler_flow_and_caching(wr90_field):
     a32 = VolumeAssembler(f, "mixed").assemble_freq_matrix(er, ur, tand, copper, 10e9).data
     assert a32.dtype.itemsize == 8
     assert rel_fro(a32.cpu().numpy().astype(complex), a64.cpu().numpy()) < 1e-6
+
+
+def test_volume_assembler_eig_matrices(wr90_field):
+    """assemble_eig_matrix's volume step (assembler.py:1115-1153): K and M converted to CSC
+    separately, conductor tets removed; equal to the oracle's."""
+    import oracle
+    from emerge_amd.assembler import VolumeAssembler
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(9)
+    er = np.zeros((3, 3, T), complex)
+    ur = np.zeros((3, 3, T), complex)
+    tand = np.zeros((3, 3, T), complex)
+    cond = np.zeros((3, 3, T), complex)
+    for d in range(3):
+        er[d, d] = rng.uniform(1, 4, T) * (1 + 0.2 * d)
+        ur[d, d] = rng.uniform(1, 2, T)
+        tand[d, d] = rng.uniform(0, 0.01, T)
+        cond[d, d] = np.where(np.arange(T) % 53 == 0, 1e9, 0.0)
+    K, M, cm, ids = VolumeAssembler(f).assemble_eig_matrices(er, ur, tand, cond, 9e9)
+    ref, ids_ref, _ = om.prepare_materials(er, tand, cond, 9e9)
+    assert np.array_equal(ids, ids_ref) and ids.size > 0
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids_ref)
+    cmap = oracle.CSCMap(rows, cols, f.n_field)
+    ex = cm.export("indptr", "indices")
+    assert np.array_equal(ex["indptr"].cpu().numpy(), cmap.indptr)
+    assert np.array_equal(ex["indices"].cpu().numpy(), cmap.indices)
+    assert rel_fro(K.cpu().numpy(), cmap.to_csc_data(dE)) <= 1e-12
+    assert rel_fro(M.cpu().numpy(), cmap.to_csc_data(dB)) <= 1e-12

@@ -227,25 +227,32 @@ __global__ __launch_bounds__(64) void csc_column_kernel(
 // key's rank is its position in the sorted order, and the outputs equal those of
 // csc_column_kernel's sort. The first key of a row (smallest local index among equal
 // rows) carries the row's entry: its unique position is the number of distinct rows
-// below it. Columns of more than kLongCap keys are refused on the host.
-constexpr int kLongCap = (int)kCscMaxColumnKeys;
+// below it. Columns of up to kLongLds keys keep their rows in LDS; longer ones (GLOBAL)
+// in a scratch slice of g_stride keys per workgroup in global memory, up to
+// kCscMaxColumnKeys (the uint16 row positions), beyond which the host refuses.
+constexpr int kLongLds = 12288;
 constexpr int kLongThreads = 256;
-static_assert(kLongCap % 64 == 0, "first-key bitmask words");
+static_assert(kLongLds % 64 == 0, "first-key bitmask words");
+static_assert(kCscMaxColumnKeys <= 65535, "row positions are uint16");
 
-template <bool FILL, int NDOF>
+template <bool FILL, int NDOF, bool GLOBAL>
 __global__ __launch_bounds__(kLongThreads) void csc_long_column_kernel(
     const int* __restrict__ t2f, const int* __restrict__ tetids,
     const long long* __restrict__ inc_ptr, const int* __restrict__ inc, long long N,
     long long* __restrict__ ucount, const long long* __restrict__ indptr,
     long long* __restrict__ indices, long long* __restrict__ perm, long long* __restrict__ seg,
-    unsigned short* __restrict__ pos) {
-    __shared__ int rows[kLongCap];
-    __shared__ unsigned long long first[kLongCap / 64];
+    unsigned short* __restrict__ pos, int* __restrict__ g_rows,
+    unsigned long long* __restrict__ g_first, int g_stride) {
+    __shared__ int s_rows[GLOBAL ? 1 : kLongLds];
+    __shared__ unsigned long long s_first[GLOBAL ? 1 : kLongLds / 64];
+    int* rows = GLOBAL ? g_rows + (size_t)blockIdx.x * g_stride : s_rows;
+    unsigned long long* first = GLOBAL ? g_first + (size_t)blockIdx.x * (g_stride / 64) : s_first;
+    const long long lo = GLOBAL ? kLongLds : kColCap, hi = GLOBAL ? g_stride : kLongLds;
     const int tid = threadIdx.x;
     for (long long c = blockIdx.x; c < N; c += gridDim.x) {
         const long long s = inc_ptr[c];
         const long long nl = NDOF * (inc_ptr[c + 1] - s);
-        if (nl <= kColCap || nl > kLongCap) continue;  // same c for the whole workgroup
+        if (nl <= lo || nl > hi) continue;  // same c for the whole workgroup
         const int n = (int)nl, nw = (n + 63) / 64;
         for (int i = tid; i < n; i += kLongThreads)
             rows[i] = t2f[tet_of(tetids, inc[s + i / NDOF] / NDOF) * NDOF + (i % NDOF)];
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(kLongThreads) void csc_long_column_kernel(
                 }
             }
         }
-        __syncthreads();  // LDS reuse by the next column
+        __syncthreads();  // row storage reuse by the next column
     }
 }
 
@@ -500,7 +507,11 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
     unsigned long long *cnt = nullptr, *maxlen = nullptr;
     int* inc = nullptr;
     long long* ucount = nullptr;
-    bool long_cols = false;
+    bool long_cols = false, huge_cols = false;
+    int* g_rows = nullptr;  // scratch rows of the columns longer than the LDS holds
+    unsigned long long* g_first = nullptr;
+    int g_stride = 0;
+    unsigned gh = 0;
 #define TRY(x)                          \
     do {                                \
         e = (x);                        \
@@ -536,12 +547,19 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
         TRY(hipMemcpyAsync(&ml, maxlen, sizeof(ml), hipMemcpyDeviceToHost, s));
         TRY(hipStreamSynchronize(s));
         max_inc = (long long)ml;
-        if (NDOF * max_inc > kLongCap) {
+        if (NDOF * max_inc > kCscMaxColumnKeys) {
             e = hipErrorInvalidValue;
             goto fail;
         }
     }
     long_cols = NDOF * max_inc > kColCap;
+    huge_cols = NDOF * max_inc > kLongLds;
+    if (huge_cols) {
+        g_stride = (int)((NDOF * max_inc + 63) / 64 * 64);
+        gh = (unsigned)(N < 512 ? N : 512);
+        TRY(hipMalloc((void**)&g_rows, sizeof(int) * (size_t)gh * g_stride));
+        TRY(hipMalloc((void**)&g_first, sizeof(unsigned long long) * (size_t)gh * (g_stride / 64)));
+    }
     // pass 1: distinct rows per column
     ucount = reinterpret_cast<long long*>(cnt);  // reuse: N+1 int64
     TRY(hipMemsetAsync(ucount, 0, sizeof(long long) * (N + 1), s));
@@ -556,9 +574,17 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
             TRY(hipGetLastError());
         }
         if (long_cols) {
-            hipLaunchKernelGGL((csc_long_column_kernel<false, NDOF>), dim3(gl),
+            hipLaunchKernelGGL((csc_long_column_kernel<false, NDOF, false>), dim3(gl),
                                dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
-                               ucount, nullptr, nullptr, nullptr, nullptr, nullptr);
+                               ucount, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, 0);
+            TRY(hipGetLastError());
+        }
+        if (huge_cols) {
+            hipLaunchKernelGGL((csc_long_column_kernel<false, NDOF, true>), dim3(gh),
+                               dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
+                               ucount, nullptr, nullptr, nullptr, nullptr, nullptr, g_rows,
+                               g_first, g_stride);
             TRY(hipGetLastError());
         }
         TRY(hipMalloc((void**)&b.indptr, sizeof(long long) * (N + 1)));
@@ -577,21 +603,33 @@ static hipError_t csc_build_n(const int* t2f, const int* tetids, long long n_ass
             TRY(hipGetLastError());
         }
         if (long_cols) {
-            hipLaunchKernelGGL((csc_long_column_kernel<true, NDOF>), dim3(gl),
+            hipLaunchKernelGGL((csc_long_column_kernel<true, NDOF, false>), dim3(gl),
                                dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
-                               nullptr, b.indptr, b.indices, b.perm, b.seg, b.pos);
+                               nullptr, b.indptr, b.indices, b.perm, b.seg, b.pos, nullptr,
+                               nullptr, 0);
+            TRY(hipGetLastError());
+        }
+        if (huge_cols) {
+            hipLaunchKernelGGL((csc_long_column_kernel<true, NDOF, true>), dim3(gh),
+                               dim3(kLongThreads), 0, s, m.t2f, m.tetids, b.inc_ptr, inc, N,
+                               nullptr, b.indptr, b.indices, b.perm, b.seg, b.pos, g_rows,
+                               g_first, g_stride);
             TRY(hipGetLastError());
         }
         TRY(hipStreamSynchronize(s));
     }
     (void)hipFree(cnt);
     (void)hipFree(maxlen);
+    (void)hipFree(g_rows);
+    (void)hipFree(g_first);
     b.inc = inc;  // kept for the incidence gather
     return hipSuccess;
 fail:
     (void)hipStreamSynchronize(s);
     (void)hipFree(cnt);
     (void)hipFree(maxlen);
+    (void)hipFree(g_rows);
+    (void)hipFree(g_first);
     (void)hipFree(inc);
     (void)hipFree(b.indptr);
     (void)hipFree(b.indices);

@@ -82,9 +82,9 @@ struct CscBuild {
     int ndof = 20;                  // DOFs per tet of the COO blocks (20 Nedelec, 10 Lagrange)
 };
 // Largest CSC column the map build takes, in COO entries (DOFs per tet x tets of the
-// column's DOF): 614 Nedelec-2 or 1,228 Lagrange-2 tets. Longer columns fail the build
-// with hipErrorInvalidValue and max_inc set.
-constexpr long long kCscMaxColumnKeys = 12288;
+// column's DOF; the row positions inside a column are uint16): 3,276 Nedelec-2 or 6,553
+// Lagrange-2 tets. Longer columns fail the build with hipErrorInvalidValue and max_inc set.
+constexpr long long kCscMaxColumnKeys = 65535;
 hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_inc,
                      hipStream_t s);
 // mirror: the COO blocks are symmetric; read each contributor from its mirrored entry.

@@ -170,7 +170,7 @@ int em_ned2_rowcol(em_ned2_plan_t plan, int64_t* rows, int64_t* cols, void* stre
  * csc_cast.py:51-53, 65-160), built on the device from the DOF map. Rows inside a
  * column are sorted ascending with duplicates merged; duplicates are summed in COO
  * index order. Blocks until the build is done (a one-time cost per mesh).
- * Fails with EM_EUNSUPPORTED if some DOF lies in more than 614 tets.
+ * Fails with EM_EUNSUPPORTED if some DOF lies in more than 3,276 tets.
  */
 int em_csc_map_create(em_csc_map_t* map, em_ned2_plan_t plan, void* stream);
 int em_csc_map_destroy(em_csc_map_t map);
@@ -374,7 +374,7 @@ int em_heat_assemble(em_heat_plan_t plan, const double* cond, int32_t cond_kind,
 /* rows/cols int64 [100 T] of the blocks (grad.py:366-370). */
 int em_heat_rowcol(em_heat_plan_t plan, int64_t* rows, int64_t* cols, void* stream);
 /* CSCMapping.from_rowcol of the heat pattern (csc_cast.py:51-53, 65-160). Blocks.
- * Fails with EM_EUNSUPPORTED if some DOF lies in more than 1,228 tets. */
+ * Fails with EM_EUNSUPPORTED if some DOF lies in more than 6,553 tets. */
 int em_csc_map_create_heat(em_csc_map_t* map, em_heat_plan_t plan, void* stream);
 /*
  * CSCMapping.to_csc for float64 data (scatter_to_csc_f8, csc_cast.py:180-194):

@@ -215,14 +215,15 @@ def test_staged_gather_non_symmetric_blocks(delaunay_field, mesh, precision):
     assert torch.equal(sub.to_csc_staged(K, M, k0, work=work), sub.to_csc_combine(K, M, k0))
 
 
-@pytest.mark.parametrize("n", [102, 103, 614])
+@pytest.mark.parametrize("n", [102, 103, 614, 615, 1200])
 def test_long_columns_and_the_limit(n):
     """A DOF in n tets gives a column of 20 n COO entries. Up to 102 tets (2,040 entries)
     the column kernel sorts them in LDS; from 103 on the long-column kernel ranks them,
-    up to 614 tets (12,280 of its 12,288 keys). Either way the pattern, perm and
-    target_csc equal the oracle's and every gather matches its CSC; a DOF in 615 tets is
-    refused with EM_EUNSUPPORTED (the reference's CPU argsort has no such limit,
-    csc_cast.py:65-160)."""
+    with the rows in LDS up to 614 tets (12,280 of 12,288 keys) and in global scratch
+    from 615 on. Either way the pattern, perm and target_csc equal the oracle's and every
+    gather matches its CSC; a DOF in 3,277 tets (more than the 65,535 entries the uint16
+    row positions index) is refused with EM_EUNSUPPORTED (the reference's CPU argsort has
+    no such limit, csc_cast.py:65-160)."""
     import torch
     from emerge_amd._lib import UnsupportedError
     from emerge_amd.assembly import CscMap
@@ -247,6 +248,6 @@ def test_long_columns_and_the_limit(n):
     assert rel_fro(got.cpu().numpy(), want) <= 1e-12
     assert torch.equal(got, cm.to_csc_combine(K, M, k0, block_symmetric=True))
     assert torch.equal(got, cm.to_csc_combine(K, M, k0, segment_gather=True))
-    if n == 614:
-        with pytest.raises(UnsupportedError, match="615 tets \\(at most 614"):
-            CscMap(_field_plan(_fan_field(615)))
+    if n == 1200:
+        with pytest.raises(UnsupportedError, match="3277 tets \\(at most 3276"):
+            CscMap(_field_plan(_fan_field(3277)))

@@ -128,13 +128,14 @@ def test_heat_csc_and_energy(wr90_field):
     assert rel_fro(Md, M.cpu().numpy()) <= 1e-14
 
 
-@pytest.mark.parametrize("n", [205, 1228])
+@pytest.mark.parametrize("n", [205, 1228, 1229, 2000])
 def test_heat_long_columns_and_the_limit(n):
     """The Lagrange-2 DOFs of a fan's axis (its two vertices and its midpoint) lie in all
     n tets: columns of 10 n COO entries, past the column kernel's 2,048-entry LDS sort
-    from 205 tets on (the long-column kernel builds them), up to 1,228 tets. Pattern and
-    values equal SciPy's COO->CSC with duplicates summed, the incidence gather equals
-    the segment gather, and 1,229 tets are refused with EM_EUNSUPPORTED."""
+    from 205 tets on (the long-column kernel builds them; rows in LDS up to 1,228 tets,
+    in global scratch beyond), up to 6,553 tets. Pattern and values equal SciPy's
+    COO->CSC with duplicates summed, the incidence gather equals the segment gather, and
+    6,554 tets are refused with EM_EUNSUPPORTED."""
     import torch
     from emerge_amd._lib import UnsupportedError
     from emerge_amd.heat import HeatCscMap, HeatPlan
@@ -159,9 +160,9 @@ def test_heat_long_columns_and_the_limit(n):
     assert rel_fro(vals.cpu().numpy(), S.data) <= 1e-14
     assert torch.equal(vals, cm.to_csc(K, segment_gather=True))
     assert torch.equal(cm.to_csc(K, M, 1e3), cm.to_csc(K, M, 1e3, segment_gather=True))
-    if n == 1228:
-        bigger = HeatPlan.from_field(Legrange2Field(_fan_field(1229).mesh))
-        with pytest.raises(UnsupportedError, match="1229 tets \\(at most 1228"):
+    if n == 2000:
+        bigger = HeatPlan.from_field(Legrange2Field(_fan_field(6554).mesh))
+        with pytest.raises(UnsupportedError, match="6554 tets \\(at most 6553"):
             HeatCscMap(bigger)
 
 

@@ -258,3 +258,7 @@ def test_prepared_material_structure_matches_the_arithmetic():
         if kind == _lib.EM_MAT_DIAG:
             assert not (np.array_equal(out[0, 0], out[1, 1]) and
                         np.array_equal(out[0, 0], out[2, 2])), name
+    # tensors already on a device are not inspected: described as full
+    import torch
+    er, tand, cond = cases["scalar"]
+    assert _prepared_kind(torch.from_numpy(er), tand, cond) == _lib.EM_MAT_FULL

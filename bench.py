@@ -72,14 +72,40 @@ def rel_err_chunked(a32, a64, chunk=1 << 26) -> float:
     return (num / den) ** 0.5
 
 
+def host_cpu_info() -> dict:
+    """The host's CPU counts: physical cores (distinct core/socket pairs of ``lscpu -p``),
+    logical CPUs (``nproc``), the CPUs this process may run on, and the thread share a
+    one-GPU job takes (at most 16 threads per GPU on a shared GPU host)."""
+    import subprocess
+    info = {"logical_cpus": os.cpu_count() or 1}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           check=True).stdout.strip())
+    except Exception:
+        info["nproc"] = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             check=True).stdout
+        info["physical_cores"] = len({ln for ln in out.splitlines()
+                                      if ln and not ln.startswith("#")})
+    except Exception:
+        info["physical_cores"] = None
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except Exception:
+        info["affinity_cpus"] = info["logical_cpus"]
+    info["job_threads"] = max(1, min(16, info["affinity_cpus"], info["physical_cores"] or 16))
+    return info
+
+
 def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int, nthreads: int = 0):
     """The oracle (C/OpenMP restatement of _matrix_builder, K and M plus int64 rows/cols
     like the reference) on the first ``chunk`` tets of the mesh, repeated for about
-    ``seconds``; returns tets/s and the thread count (default: up to 16, the host share of
-    one GPU)."""
+    ``seconds``; returns tets/s and the thread count (default: the job's thread share,
+    ``host_cpu_info()["job_threads"]``)."""
     import numpy as np
     import oracle
-    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    nthreads = nthreads or host_cpu_info()["job_threads"]
     m = field.mesh
     ids = np.arange(min(chunk, field.n_tets), dtype=np.int64)
     args = (m.nodes, m.tets, m.tris, m.edges, field.tet_to_field, m.tet_to_edge, m.tet_to_tri,
@@ -675,12 +701,18 @@ def main() -> int:
         for d in range(3):
             er_ref[d, d] = eps_np
             ur_ref[d, d] = mu_np[:, d]
-        log("cpu baseline (oracle) ...")
-        tps, nth, done, el = cpu_baseline(field, er_ref, ur_ref, args.cpu_seconds, args.cpu_chunk)
-        cpu = {"value": tps, "unit": "tets/s", "cores": nth, "kind": "port",
+        host = host_cpu_info()
+        log(f"cpu baseline (oracle) ... host {host}")
+        tps, nth, done, el = cpu_baseline(field, er_ref, ur_ref, args.cpu_seconds, args.cpu_chunk,
+                                          nthreads=host["job_threads"])
+        cpu = {"value": tps, "unit": "tets/s", "cores": nth, "threads": nth, "kind": "port",
                "sample": f"first {min(args.cpu_chunk, T)} tets of the same mesh, repeated "
                          f"({done} tets in {el:.1f}s); K and M complex128 + int64 rows/cols "
-                         f"as the reference's _matrix_builder"}
+                         f"as the reference's _matrix_builder",
+               "host_physical_cores": host["physical_cores"], "host_nproc": host["nproc"],
+               "host_affinity_cpus": host["affinity_cpus"],
+               "threads_note": "one OpenMP thread per core used; the job's share of the host "
+                               "is at most 16 threads per GPU"}
         log(f"cpu: {tps:.0f} tets/s on {nth} threads")
         # the reference's default thread count (NUMBA_NUM_THREADS=4)
         tps4, _, done4, el4 = cpu_baseline(field, er_ref, ur_ref, args.cpu_seconds / 2,

@@ -81,7 +81,7 @@ class VolumeAssembler:
         self.precision = precision
         self.peclim = peclim
         self.surfimplim = surfimplim
-        self.cached_matrices = None  # (plan, K, M)
+        self.cached_matrices = None  # (plan, K, M, blocks symmetric)
         self._map = None  # (plan, CscMap)
         self.assemblies = 0
 
@@ -97,13 +97,16 @@ class VolumeAssembler:
         fd = fd or frequency_dependent
         c = self.cached_matrices
         if cache_matrices and not fd and c is not None and c[0] is plan:
-            K, M = c[1], c[2]
+            # the symmetry of the cached blocks is that of the materials they were built
+            # from, not of this call's
+            K, M, sym = c[1], c[2], c[3]
         else:
             K, M = plan.assemble_km(er_m, ur_m, self.precision)
+            sym = _symmetric(er_m, ur_m)
             self.assemblies += 1
-            self.cached_matrices = (plan, K, M)
+            self.cached_matrices = (plan, K, M, sym)
         cm = self._cscmap(plan)
-        data = cm.to_csc_combine(K, M, K0, block_symmetric=_symmetric(er_m, ur_m))
+        data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
         return FreqMatrix(data, cm, conductor_tets, fd, K0)
 
     def assemble_eig_matrices(self, er, ur, tand, cond, frequency: float):
@@ -129,7 +132,9 @@ class VolumeAssembler:
         T = plan.n_tets
         kind = _prepared_kind(er, tand, cond)
         comp = {_lib.EM_MAT_FULL: T, _lib.EM_MAT_DIAG: 4 * T, _lib.EM_MAT_SCALAR: 0}[kind]
-        return plan, Material(er_d, kind, 1, comp), plan.material(ur), conductor_tets, fd
+        # ur is a (3, 3, T) tensor whether it lives on the host or on the device
+        ur_m = plan.material(ur, layout="ref")
+        return plan, Material(er_d, kind, 1, comp), ur_m, conductor_tets, fd
 
     def _cscmap(self, plan) -> CscMap:
         """The CSC map of ``plan`` (the reference's ``cached_cscmap``)."""

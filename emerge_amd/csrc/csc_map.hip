@@ -378,11 +378,12 @@ __global__ __launch_bounds__(256) void csc_gather_kernel(const long long* __rest
 // lanes (lane ii of a group handles row slot ii); a group walks its column's incidence
 // list in order, loads the incidence's 20 values (with MIRROR one contiguous 20-entry
 // row of the block) and adds each into the column's FP64 accumulator in LDS at pos[]
-// (the entry index of the row within the column). LDS operations of a wavefront complete
-// in issue order and the 20 lanes of one incidence hit distinct entries, so every entry
-// sums its contributors in incidence order = ascending COO index, as the perm/seg gather
-// does: the result is the same bit for bit. Columns longer than kIncCap entries are done
-// in windows.
+// (the entry index of the row within the column). The 20 lanes of one incidence hit
+// distinct entries and a group issues its incidences in list order, so every entry sums
+// its contributors in incidence order = ascending COO index, as the perm/seg gather does:
+// the result is the same bit for bit. The zeroing, the adds and the read-back of a column
+// window are separated by wavefront LDS syncs (lanes hand entries to each other through
+// LDS). Columns longer than kIncCap entries are done in windows.
 constexpr int kIncCap = 128;
 constexpr int kIncWaves = 4;  // wavefronts per workgroup
 constexpr int kIncCols = 3;   // columns per wavefront
@@ -433,6 +434,7 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
                 a[p][0] = 0.0;
                 a[p][1] = 0.0;
             }
+            wave_lds_sync();  // zeroed accumulators before any lane adds
             // batches of kU incidences: all loads of a batch are issued before the adds
             constexpr int kU = 4;
             for (long long i0 = 0; i0 < ninc; i0 += kU) {
@@ -463,8 +465,10 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_kernel(
                         atomicAdd(&a[p[u]][1], v[u].y);
                     }
             }
+            wave_lds_sync();  // every lane's adds before the read-back
             for (int p = ii; p < wl; p += 20) store_c<TO>(out, cb + w0 + p,
                                                          make_double2(a[p][0], a[p][1]));
+            wave_lds_sync();  // read-back before the next window or column zeroes
         }
     }
 }
@@ -682,6 +686,7 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_real_kernel(
         for (long long w0 = 0; w0 < len; w0 += kIncCap) {
             const int wl = (int)(len - w0 < kIncCap ? len - w0 : kIncCap);
             for (int p = ii; p < wl; p += NDOF) a[p] = 0.0;
+            wave_lds_sync();  // zeroed accumulators before any lane adds
             constexpr int kU = 4;
             for (long long i0 = 0; i0 < ninc; i0 += kU) {
                 int p[kU];
@@ -701,7 +706,9 @@ __global__ __launch_bounds__(64 * kIncWaves) void csc_gather_inc_real_kernel(
                 for (int u = 0; u < kU; ++u)
                     if (p[u] >= 0 && p[u] < wl) atomicAdd(&a[p[u]], v[u]);
             }
+            wave_lds_sync();  // every lane's adds before the read-back
             for (int p = ii; p < wl; p += NDOF) out[cb + w0 + p] = a[p];
+            wave_lds_sync();  // read-back before the next window or column zeroes
         }
     }
 }
@@ -819,17 +826,13 @@ __global__ __launch_bounds__(256) void coo_stage_transpose_kernel(const T* __res
                 tile[wave][20 * jj + ii] = contrib<T>(E, HAS_B ? B : nullptr, k0sq, base + e);
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_lds_sync();
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
             const int e = lane + 64 * k;
             if (e < 400) work[base + e] = tile[wave][e];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_lds_sync();
     }
 }
 

@@ -31,6 +31,15 @@ struct MeshDev {
 
 enum OutMode { MODE_A = 0, MODE_KM = 1, MODE_AF = 2 };
 
+// Hand-off of LDS data between the lanes of one wavefront: every lane's earlier LDS
+// accesses are complete and visible before any lane's later ones (wavefront-scope
+// release and acquire fences around a wave barrier; no workgroup barrier needed).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Wavenumbers of one frequency-sweep launch, passed by value as a kernel argument.
 constexpr int kMaxK0 = 128;
 struct K0Pack {

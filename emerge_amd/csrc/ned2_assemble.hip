@@ -17,12 +17,16 @@
 // s = sign(det of the stored-order Jacobian): the reference normalises barycentric
 // coefficients by |V| (mth/optimized.py:142-167, curlcurl.py:306-311).
 //
-// Work layout: one workgroup of 256 threads owns a tile of 64 consecutive assembly
-// indices. Phase 1: wave 0 computes the metric tensors of the 64 tets (FP64) into LDS.
-// Phase 2: thread j < 210 owns canonical pair j (a <= b) with its 12 (18) tensor
-// coefficients in registers, and for each sub-batch of NB tets writes its value into
-// both mirrored positions of an LDS copy of the 20x20 blocks. Phase 3: all 256 threads
-// stream the sub-batch (NB*400 contiguous COO entries) to HBM with 16-byte stores.
+// Work layout: the plan groups assembly indices by vertex-rank code inside windows of
+// consecutive tets and cuts each group into tiles of 32, 64 or 128 tets (autotuned), all
+// of one code, interleaved so that the tiles of a window run on one XCD. One workgroup
+// of 256 threads per tile. Phase 1: threads < tile compute one tet's metric tensors
+// (FP64; materials gathered through strides) into LDS. Phase 2: threads 0..199 own two
+// output entries of every 20x20 block, with the reference-tensor coefficients of their
+// two canonical pairs in registers; per tet of the tile they read the metric tensors
+// from LDS (broadcast), contract, and store their 16 (complex64) or 32 (complex128)
+// bytes straight to the COO arrays with non-temporal 16-byte stores, contiguous across
+// the wavefront. One barrier per tile; the output is not staged in LDS.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

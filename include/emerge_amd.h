@@ -19,9 +19,10 @@
  *   nodes (3, Nn) f8, tets (4, T), tris (3, F), edges (2, E), tet_to_field (20, T),
  *   tet_to_edge (6, T), tet_to_tri (4, T), tetids (Ta,), dofcodes (20,).
  * Host pointers are used for the mesh tables given to em_ned2_plan_create (copied to
- * the device once). Material tensors, k0 values and every output are DEVICE pointers;
- * launches are asynchronous on the given HIP stream (hipStream_t passed as void*,
- * NULL = default stream).
+ * the device once) and for the k0 array of em_ned2_assemble_a (its values travel as
+ * kernel arguments). Material tensors and every output are DEVICE pointers; launches are
+ * asynchronous on the given HIP stream (hipStream_t passed as void*, NULL = default
+ * stream). Functions that say "Blocks" return after their device work is done.
  *
  * COO output layout (identical to curlcurl.py:449-455): the 20x20 block of assembly
  * index ia (tet tetids[ia]) occupies entries [400*ia, 400*ia + 400), row-major

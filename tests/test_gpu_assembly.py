@@ -126,3 +126,57 @@ def test_mixed_precision_vs_fp64(delaunay_field, eps_kind, mu_kind):
     K32, M32 = plan.assemble_km(er, ur, "mixed")
     assert rel_fro(K32.cpu().numpy(), K64.cpu().numpy()) <= MIXED_TOL
     assert rel_fro(M32.cpu().numpy(), M64.cpu().numpy()) <= MIXED_TOL
+
+
+def test_assemble_a_through_raw_ctypes(wr90_field):
+    """em_ned2_assemble_a called the way a C caller would (include/emerge_amd.h): host mesh
+    tables to em_ned2_plan_create, device material descriptors, a HOST k0 array of three
+    wavenumbers and a strided device output; each frequency's COO values equal the
+    oracle's dataE - k0^2 dataB."""
+    import ctypes
+
+    import torch
+
+    from emerge_amd import _lib
+    L = _lib.lib()
+    f = wr90_field
+    m = f.mesh
+    T = f.n_tets
+    rng = np.random.default_rng(21)
+    er = ref_tensor(T, "scalar", rng)
+    ur = ref_tensor(T, "diag", rng)
+    keep = []
+
+    def hp(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    plan = ctypes.c_void_p()
+    st = L.em_ned2_plan_create(ctypes.byref(plan), 0, hp(m.nodes, np.float64), m.n_nodes,
+                               hp(m.tets, np.int64), T, hp(m.tris, np.int64), m.n_tris,
+                               hp(m.edges, np.int64), m.n_edges, hp(f.tet_to_field, np.int64),
+                               hp(m.tet_to_edge, np.int64), hp(m.tet_to_tri, np.int64),
+                               f.n_field, None, 0, hp(f.dofcodes3d, np.int64), 20)
+    assert st == _lib.EM_OK, L.em_last_error()
+    try:
+        er_d = torch.as_tensor(np.ascontiguousarray(er), device="cuda")
+        ur_d = torch.as_tensor(np.ascontiguousarray(ur), device="cuda")
+        # the reference's (3, 3, T) arrays: full tensors, tet stride 1, component stride T
+        e = _lib.em_material_t(ctypes.c_void_p(er_d.data_ptr()), _lib.EM_MAT_FULL, 0, 1, T)
+        u = _lib.em_material_t(ctypes.c_void_p(ur_d.data_ptr()), _lib.EM_MAT_FULL, 0, 1, T)
+        k0 = (ctypes.c_double * 3)(*(2 * np.pi * np.array([8.5e9, 10e9, 11.5e9]) / 299792458))
+        stride = 400 * T + 64  # whole-line padding between the frequencies
+        out = torch.zeros(3 * stride, dtype=torch.complex128, device="cuda")
+        st = L.em_ned2_assemble_a(plan, ctypes.byref(e), ctypes.byref(u), k0, 3,
+                                  ctypes.c_void_p(out.data_ptr()), stride, _lib.EM_PREC_FP64,
+                                  None)
+        assert st == _lib.EM_OK, L.em_last_error()
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+    finally:
+        L.em_ned2_plan_destroy(plan)
+    dE, dB, _, _ = _oracle(f, er, ur, local_frame=False)
+    for i in range(3):
+        assert rel_fro(got[i * stride:i * stride + 400 * T], dE - dB * k0[i] ** 2) <= FP64_TOL
+        assert not np.any(got[i * stride + 400 * T:(i + 1) * stride])  # padding untouched

@@ -173,3 +173,81 @@ def test_volume_assembler_eig_matrices(wr90_field):
     assert np.array_equal(ex["indices"].cpu().numpy(), cmap.indices)
     assert rel_fro(K.cpu().numpy(), cmap.to_csc_data(dE)) <= 1e-12
     assert rel_fro(M.cpu().numpy(), cmap.to_csc_data(dB)) <= 1e-12
+
+
+def _device_inputs(T, seed):
+    """er, ur, tand, cond as complex (3, 3, T) arrays: lossy anisotropic er, a full
+    non-symmetric ur, a few copper tets."""
+    rng = np.random.default_rng(seed)
+    er = rng.normal(size=(3, 3, T)) * 0.2 + 0j
+    ur = rng.normal(size=(3, 3, T)) * 0.3 + 1j * rng.normal(size=(3, 3, T)) * 0.05
+    tand = np.zeros((3, 3, T), complex)
+    cond = np.zeros((3, 3, T), complex)
+    for d in range(3):
+        er[d, d] += rng.uniform(2, 4, T)
+        ur[d, d] += 2.0
+        tand[d, d] = rng.uniform(0, 0.01, T)
+        cond[d, d] = np.where(np.arange(T) % 41 == 5, 5.8e7, 0.0)
+    return er, ur, tand, cond
+
+
+@pytest.mark.parametrize("which", ["wr90", "three_tets"])
+def test_volume_assembler_device_inputs(wr90_field, which):
+    """er, ur, tand and cond all handed over as CUDA (3, 3, T) tensors, as _prepare's
+    docstring allows: assemble_freq_matrix and assemble_eig_matrices equal the oracle on
+    the same arrays. The three-tet mesh is the case where a (3, 3, T) device tensor has the
+    shape of a tet-major (T, 3, 3) one."""
+    import torch
+
+    import oracle
+    from emerge_amd.assembler import VolumeAssembler
+    from emerge_amd.mesh import Nedelec2Field, TetMesh, hex_box_mesh
+    if which == "wr90":
+        f = wr90_field
+    else:
+        hm = hex_box_mesh(1e-2, 1e-2, 1e-2, 1, 1, 1)
+        f = Nedelec2Field(TetMesh(hm.nodes, hm.tets[:, :3]))
+    T = f.n_tets
+    er, ur, tand, cond = _device_inputs(T, 12)
+    dev = [torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in (er, ur, tand, cond)]
+    freq = 9.1e9
+    k0 = 2 * np.pi * freq / 299792458
+    ref, ids, fd = om.prepare_materials(er, tand, cond, freq)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids)
+    cmap = oracle.CSCMap(rows, cols, f.n_field)
+    r = VolumeAssembler(f).assemble_freq_matrix(*dev, freq)
+    assert np.array_equal(r.conductor_tets, ids) and r.is_frequency_dependent == fd
+    A = r.to_scipy()
+    assert np.array_equal(A.indptr, cmap.indptr) and np.array_equal(A.indices, cmap.indices)
+    assert rel_fro(A.data, cmap.to_csc_data(dE - dB * k0 ** 2)) <= 1e-12
+    K, M, cm, ids_e = VolumeAssembler(f).assemble_eig_matrices(*dev, freq)
+    assert np.array_equal(ids_e, ids)
+    assert rel_fro(K.cpu().numpy(), cmap.to_csc_data(dE)) <= 1e-12
+    assert rel_fro(M.cpu().numpy(), cmap.to_csc_data(dB)) <= 1e-12
+
+
+def test_volume_assembler_cached_blocks_keep_their_symmetry(wr90_field):
+    """Cached K/M reused with materials of another structure: the CSC combine reads the
+    cached blocks as the materials they were built from require (non-symmetric blocks of
+    a full er are never read mirrored because a later call passes diagonal tensors)."""
+    import oracle
+    from emerge_amd.assembler import VolumeAssembler
+    f = wr90_field
+    T = f.n_tets
+    er, ur, tand, cond = _device_inputs(T, 13)
+    cond[:] = 0
+    freq1, freq2 = 8.7e9, 10.3e9
+    va = VolumeAssembler(f)
+    va.assemble_freq_matrix(er, ur, tand, cond, freq1)
+    # same plan (no conductors), nothing frequency dependent: the blocks are reused
+    er_d = np.zeros_like(er)
+    ur_d = np.zeros_like(ur)
+    for d in range(3):
+        er_d[d, d], ur_d[d, d] = er[d, d], ur[d, d]
+    r = va.assemble_freq_matrix(er_d, ur_d, tand, cond, freq2)
+    assert va.assemblies == 1
+    ref, ids, _ = om.prepare_materials(er, tand, cond, freq1)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids)
+    k0 = 2 * np.pi * freq2 / 299792458
+    want = oracle.CSCMap(rows, cols, f.n_field).to_csc_data(dE - dB * k0 ** 2)
+    assert rel_fro(r.data.cpu().numpy(), want) <= 1e-12

@@ -36,6 +36,7 @@ EM_INTERP_CURL = 1
 
 EM_PREC_FP64 = 0
 EM_PREC_MIXED = 1
+EM_PREC_FP64_REF = 2
 
 
 class EmError(RuntimeError):

@@ -18,14 +18,23 @@ from ._lib import check, em_material_t
 
 PREC_FP64 = _lib.EM_PREC_FP64
 PREC_MIXED = _lib.EM_PREC_MIXED
-_PREC = {"fp64": PREC_FP64, "mixed": PREC_MIXED, PREC_FP64: PREC_FP64, PREC_MIXED: PREC_MIXED}
+PREC_FP64_REF = _lib.EM_PREC_FP64_REF
+_PREC = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "fp64_ref": PREC_FP64_REF,
+         PREC_FP64: PREC_FP64, PREC_MIXED: PREC_MIXED, PREC_FP64_REF: PREC_FP64_REF}
 
 
 def _prec(p) -> int:
+    """Precision name or code: ``"fp64"`` (exact element integrals, complex128),
+    ``"mixed"`` (FP32 contraction, complex64) or ``"fp64_ref"`` (the reference's own
+    11-point arithmetic in absolute coordinates, complex128; assembly only)."""
     try:
         return _PREC[p]
     except KeyError:
-        raise ValueError(f"unknown precision {p!r}; use 'fp64' or 'mixed'") from None
+        raise ValueError(f"unknown precision {p!r}; use 'fp64', 'mixed' or 'fp64_ref'") from None
+
+
+def _out_dtype(prec: int):
+    return torch.complex64 if prec == PREC_MIXED else torch.complex128
 
 
 def _stream_ptr(stream) -> ctypes.c_void_p:
@@ -226,7 +235,7 @@ class Ned2Plan:
         """K (stiffness, mu_r^-1) and M (mass, eps_r) COO value arrays, reference layout."""
         prec = _prec(precision)
         er_m, ur_m = self.material(er), self.material(ur)
-        dt = torch.complex128 if prec == PREC_FP64 else torch.complex64
+        dt = _out_dtype(prec)
         if out is None:
             K = torch.empty(self.nnz_coo, dtype=dt, device=self.device)
             M = torch.empty(self.nnz_coo, dtype=dt, device=self.device)
@@ -246,7 +255,7 @@ class Ned2Plan:
         sequence of wavenumbers (returns ``[len(k0), nnz_coo]``, one launch)."""
         prec = _prec(precision)
         er_m, ur_m = self.material(er), self.material(ur)
-        dt = torch.complex128 if prec == PREC_FP64 else torch.complex64
+        dt = _out_dtype(prec)
         scalar = np.isscalar(k0)
         k0a = np.ascontiguousarray(np.atleast_1d(np.asarray(k0, dtype=np.float64)))
         nf = int(k0a.shape[0])

@@ -392,7 +392,8 @@ int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er, const em_m
     int st;
     if ((st = check_material(er, plan->n_tets, "er")) || (st = check_material(ur, plan->n_tets, "ur")))
         return st;
-    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED &&
+        precision != EM_PREC_FP64_REF) {
         set_error("unknown precision");
         return EM_EINVAL;
     }
@@ -405,6 +406,12 @@ int em_ned2_assemble_km(em_ned2_plan_t plan, const em_material_t* er, const em_m
         return EM_EINVAL;
     }
     DeviceGuard g(plan->device);
+    if (precision == EM_PREC_FP64_REF) {
+        EM_HIP(em::launch_assemble_ref(plan->dev(), to_desc(er, false), to_desc(ur, true),
+                                       em::MODE_KM, nullptr, 0, dataE, dataB, 0,
+                                       (hipStream_t)stream));
+        return EM_OK;
+    }
     em::TileSet ts[3];
     plan->tilesets(ts);
     EM_HIP(em::launch_assemble(plan->dev(), ts, plan->tile, to_desc(er, false), to_desc(ur, true),
@@ -423,7 +430,8 @@ int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er, const em_ma
     int st;
     if ((st = check_material(er, plan->n_tets, "er")) || (st = check_material(ur, plan->n_tets, "ur")))
         return st;
-    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED &&
+        precision != EM_PREC_FP64_REF) {
         set_error("unknown precision");
         return EM_EINVAL;
     }
@@ -452,6 +460,12 @@ int em_ned2_assemble_a(em_ned2_plan_t plan, const em_material_t* er, const em_ma
         k0sq[f] = k0[f] * k0[f];
     }
     DeviceGuard g(plan->device);
+    if (precision == EM_PREC_FP64_REF) {
+        EM_HIP(em::launch_assemble_ref(plan->dev(), to_desc(er, false), to_desc(ur, true),
+                                       n_k0 == 1 ? em::MODE_A : em::MODE_AF, k0sq.data(), n_k0,
+                                       out, nullptr, freq_stride, (hipStream_t)stream));
+        return EM_OK;
+    }
     em::TileSet ts[3];
     plan->tilesets(ts);
     EM_HIP(em::launch_assemble(plan->dev(), ts, plan->tile, to_desc(er, false), to_desc(ur, true),

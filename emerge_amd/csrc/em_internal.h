@@ -65,6 +65,11 @@ hipError_t launch_assemble(const MeshDev& m, const TileSet* ts, int tile, const 
                            const MatDesc& mu, int mode, int precision, int full,
                            const double* k0sq, int nf, void* out0, void* out1,
                            long long freq_stride, hipStream_t stream);
+// The reference's own arithmetic (ned2_ref.hip, EM_PREC_FP64_REF): 11-point rule in
+// absolute coordinates, complex128 out; modes as launch_assemble.
+hipError_t launch_assemble_ref(const MeshDev& m, const MatDesc& eps, const MatDesc& mu, int mode,
+                               const double* k0sq, int nf, void* out0, void* out1,
+                               long long freq_stride, hipStream_t stream);
 hipError_t launch_rowcol(const MeshDev& m, long long* rows, long long* cols, hipStream_t s);
 hipError_t launch_check_tables(const int4* tets, long long T, const long long* edges,
                                long long E, const long long* tris, long long F,

@@ -107,15 +107,24 @@ class CSCMapping:
 
 
 def tet_mass_stiffness_matrices(field, er: np.ndarray, ur: np.ndarray,
-                                conductor_tets: np.ndarray, cscmap: CSCMapping | None = None):
+                                conductor_tets: np.ndarray, cscmap: CSCMapping | None = None,
+                                precision: str = "fp64"):
     """GPU drop-in for ``tet_mass_stiffness_matrices`` (curlcurl.py:222-270).
 
     Returns ``(dataE, dataB, cscmap)`` with ``dataE`` the stiffness (``mu_r^-1``) and
     ``dataB`` the mass (``eps_r``) COO values, complex128 NumPy arrays of length
     ``400 * (T - len(conductor_tets))`` in the reference's layout.
+
+    ``precision`` (not in the reference's signature; the default keeps its calls
+    unchanged): ``"fp64"`` gives the exact element integrals; ``"fp64_ref"`` repeats the
+    reference's own 11-point arithmetic in absolute coordinates, whose cofactor sums
+    cancel on thin tets (the two differ by up to ~1e-7 relative on seeded Delaunay
+    meshes, ~1e-15 on well-shaped meshes near the origin).
     """
+    if precision not in ("fp64", "fp64_ref"):
+        raise ValueError("the drop-in returns complex128: precision 'fp64' or 'fp64_ref'")
     plan = plan_for(field, conductor_tets)
-    K, M = plan.assemble_km(er, ur, "fp64")
+    K, M = plan.assemble_km(er, ur, precision)
     if cscmap is None:
         cscmap = CSCMapping.from_plan(plan)
     return K.cpu().numpy(), M.cpu().numpy(), cscmap

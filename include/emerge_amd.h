@@ -70,6 +70,17 @@ typedef struct {
 /* ---- precision / output type ---- */
 #define EM_PREC_FP64 0   /* FP64 geometry and contraction, complex128 output */
 #define EM_PREC_MIXED 1  /* FP64 geometry, FP32 contraction, complex64 output */
+/*
+ * The reference's own FP64 arithmetic, complex128 output (em_ned2_assemble_km and
+ * em_ned2_assemble_a only): the 11-point rule of curlcurl.py:277-380 in absolute
+ * coordinates, with tet_coefficients (curlcurl.py:125-213), matinv
+ * (mth/optimized.py:315-357) and the compiled/ccbf.py basis functions evaluated in the
+ * reference's operand order without fused multiply-adds. EM_PREC_FP64 computes the exact
+ * element integrals from vertex differences instead; on thin tets far from the origin the
+ * reference's cofactor sums cancel and the two differ (up to ~1e-7 relative on seeded
+ * Delaunay meshes). Compute-bound and several times slower than EM_PREC_FP64.
+ */
+#define EM_PREC_FP64_REF 2
 
 typedef struct em_ned2_plan_s* em_ned2_plan_t;
 typedef struct em_csc_map_s* em_csc_map_t;

@@ -7,24 +7,23 @@
  * product path never calls it.
  *
  * Followed step by step (paths relative to src/emerge/_emerge/ in the reference):
- *   or_tet_coefficients   mth/optimized.py:127-214   cofactors of the 4x4 vertex
- *                         matrix and V = |det|/6 (the |V| quirk: coeff = cof/(6V)
- *                         is sign(det) times the true barycentric coefficients)
+ *   or_ref_tet_coefficients, or_ref_matinv, or_ref_coeff, or_ref_eval_point
+ *                         (ned2_strict.c): the steps Numba compiles without fastmath
+ *                         (tet_coefficients curlcurl.py:125-213, matinv
+ *                         mth/optimized.py:315-357, the ccbf.py 3D basis functions and
+ *                         curls of codes 64/66/128/129), in the reference's own
+ *                         evaluation order, compiled without fast-math or contraction
  *   or_local_map          mth/optimized.py:262-296, physics/.../curlcurl.py:72-85
- *   or_matinv             mth/optimized.py:315-357 (diagonal fast path kept)
- *   basis / curl          compiled/ccbf.py 3D functions for codes 64 (edge fn 0),
- *                         66 (edge fn 2), 128 (face fn 0), 129 (face fn 1):
- *                         _ne0_3d :536, _ne2_3d :568, _nf0_3d :789, _nf1_3d :807,
- *                         _curl_ne0_3d :643, _curl_ne2_3d :658, _curl_nf0_3d :927,
- *                         _curl_nf1_3d :948; written here from their definitions
- *                         (elements/dofsets.py:66-83), curls derived by hand
  *   or_parse_dofcode      compiled/ccbf.py:40-66
  *   or_tet_stiff_mass     physics/microwave/assembly/curlcurl.py:285-380 (11-point
- *                         Keast rule of :277-283, complex arithmetic, i<=j then mirror)
+ *                         Keast rule of :277-283, complex arithmetic, i<=j then mirror;
+ *                         the material products, weights and sums are fastmath in the
+ *                         reference and compiled here with -ffast-math)
  *   ned2_oracle_build     curlcurl.py:387-456 (_matrix_builder; COO block per tet,
  *                         row-major, p = iassy*ndof^2)
  *   ned2_oracle_csc_*     mth/csc_cast.py:65-177 (precompute_csc_pattern, scatter)
- *   ned2_oracle_interp    compiled/base/interp.py:286-387, 410-519 (field and curl)
+ *   ned2_oracle_interp    compiled/base/interp.py:286-387, 410-519 (field and curl;
+ *                         basis written from the definitions in elements/dofsets.py:66-83)
  *   ned2_oracle_locate    compiled/base/index_inter.py:134-246 (brute force)
  */
 #include <complex.h>
@@ -35,58 +34,20 @@
 
 typedef double _Complex cplx;
 
-/* 11-point Keast degree-4 rule: weights then barycentric points
- * (curlcurl.py:277-283, identical to keast_rules.py KEAST4). */
-static const double DPTS[5][11] = {
-    {-0.07893333333333329982, 0.04573333333333329948, 0.04573333333333329948,
-     0.04573333333333329948, 0.04573333333333329948, 0.14933333333333329018,
-     0.14933333333333329018, 0.14933333333333329018, 0.14933333333333329018,
-     0.14933333333333329018, 0.14933333333333329018},
-    {0.25000000000000000000, 0.78571428571428569843, 0.07142857142857139685,
-     0.07142857142857139685, 0.07142857142857139685, 0.10059642383320080428,
-     0.39940357616679922348, 0.39940357616679922348, 0.39940357616679922348,
-     0.10059642383320080428, 0.10059642383320080428},
-    {0.25000000000000000000, 0.07142857142857139685, 0.07142857142857139685,
-     0.07142857142857139685, 0.78571428571428569843, 0.39940357616679922348,
-     0.10059642383320080428, 0.39940357616679922348, 0.10059642383320080428,
-     0.39940357616679922348, 0.10059642383320080428},
-    {0.25000000000000000000, 0.07142857142857139685, 0.07142857142857139685,
-     0.78571428571428569843, 0.07142857142857139685, 0.39940357616679922348,
-     0.39940357616679922348, 0.10059642383320080428, 0.10059642383320080428,
-     0.10059642383320080428, 0.39940357616679922348},
-    {0.25000000000000000000, 0.07142857142857150787, 0.78571428571428580945,
-     0.07142857142857150787, 0.07142857142857150787, 0.10059642383320077652,
-     0.10059642383320077652, 0.10059642383320074877, 0.39940357616679922348,
-     0.39940357616679922348, 0.39940357616679922348},
-};
+/* ned2_strict.c: the quadrature table and the steps the reference evaluates strictly */
+extern const double OR_DPTS[5][11];
+#define DPTS OR_DPTS
+void or_ref_tet_coefficients(const double x[4], const double y[4], const double z[4],
+                             double a[4], double b[4], double c[4], double d[4], double *V);
+void or_ref_matinv(const cplx M[9], cplx out[9]);
+void or_ref_coeff(const double a[4], const double b[4], const double c[4], const double d[4],
+                  double V, double coeff[4][4]);
+int or_ref_eval_point(const double coeff[4][4], const double vx[3][4], int q,
+                      const int64_t *codes, const int (*ijk)[3], int ndof, double (*F)[3],
+                      double (*CF)[3]);
 #define NQ 11
 
 enum { OR_OK = 0, OR_EBADCODE = -2, OR_EDEGEN = -5 };
-
-static double det3(double a0, double a1, double a2, double b0, double b1, double b2,
-                   double c0, double c1, double c2) {
-    return a0 * (b1 * c2 - b2 * c1) - a1 * (b0 * c2 - b2 * c0) + a2 * (b0 * c1 - b1 * c0);
-}
-
-/* Cofactors C[n][k] of P = [[1,x_n,y_n,z_n]] (row n, column k) and |det P|/6.
- * a_n = C[n][0], b_n = C[n][1], c_n = C[n][2], d_n = C[n][3]. */
-static void or_tet_coefficients(const double x[4], const double y[4], const double z[4],
-                                double a[4], double b[4], double c[4], double d[4],
-                                double *V) {
-    for (int n = 0; n < 4; ++n) {
-        int r[3], m = 0;
-        for (int k = 0; k < 4; ++k)
-            if (k != n) r[m++] = k;
-        double sg = (n % 2 == 0) ? 1.0 : -1.0;
-        a[n] = sg * det3(x[r[0]], y[r[0]], z[r[0]], x[r[1]], y[r[1]], z[r[1]],
-                         x[r[2]], y[r[2]], z[r[2]]);
-        b[n] = -sg * det3(1.0, y[r[0]], z[r[0]], 1.0, y[r[1]], z[r[1]], 1.0, y[r[2]], z[r[2]]);
-        c[n] = sg * det3(1.0, x[r[0]], z[r[0]], 1.0, x[r[1]], z[r[1]], 1.0, x[r[2]], z[r[2]]);
-        d[n] = -sg * det3(1.0, x[r[0]], y[r[0]], 1.0, x[r[1]], y[r[1]], 1.0, x[r[2]], y[r[2]]);
-    }
-    /* det P = sum_n C[n][0] (expansion along the column of ones) */
-    *V = fabs(a[0] + a[1] + a[2] + a[3]) / 6.0;
-}
 
 /* Local vertex index of every global id of a small table column (optimized.py:262). */
 static void or_local_map(const int64_t vid[4], const int64_t *gids, int nrow, int64_t *out) {
@@ -95,27 +56,6 @@ static void or_local_map(const int64_t vid[4], const int64_t *gids, int nrow, in
         for (int k = 0; k < 4; ++k)
             if (vid[k] == gids[j]) { out[j] = k; break; }
     }
-}
-
-static void or_matinv(const cplx M[9], cplx out[9]) {
-    if (M[1] == 0 && M[2] == 0 && M[3] == 0 && M[5] == 0 && M[6] == 0 && M[7] == 0) {
-        for (int i = 0; i < 9; ++i) out[i] = 0;
-        out[0] = 1.0 / M[0];
-        out[4] = 1.0 / M[4];
-        out[8] = 1.0 / M[8];
-        return;
-    }
-    cplx det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
-               M[2] * (M[3] * M[7] - M[4] * M[6]);
-    out[0] = (M[4] * M[8] - M[5] * M[7]) / det;
-    out[1] = (M[2] * M[7] - M[1] * M[8]) / det;
-    out[2] = (M[1] * M[5] - M[2] * M[4]) / det;
-    out[3] = (M[5] * M[6] - M[3] * M[8]) / det;
-    out[4] = (M[0] * M[8] - M[2] * M[6]) / det;
-    out[5] = (M[2] * M[3] - M[0] * M[5]) / det;
-    out[6] = (M[3] * M[7] - M[4] * M[6]) / det;
-    out[7] = (M[1] * M[6] - M[0] * M[7]) / det;
-    out[8] = (M[0] * M[4] - M[1] * M[3]) / det;
 }
 
 static inline void cross3(const double u[3], const double v[3], double w[3]) {
@@ -190,51 +130,33 @@ static int or_tet_stiff_mass(const double vx[3][4], const int64_t lem[2][6],
                              const int64_t *codes, const int64_t *typ, const int64_t *idx,
                              int ndof, cplx *Kout, cplx *Mout) {
     double a[4], b[4], c[4], d[4], V;
-    or_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
+    or_ref_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
     if (!(V > 0.0)) return OR_EDEGEN;
     double co[4][4]; /* co[0]=a/(6V), co[1]=b/(6V) ... (curlcurl.py:307-311) */
-    for (int n = 0; n < 4; ++n) {
-        co[0][n] = a[n] / (6.0 * V);
-        co[1][n] = b[n] / (6.0 * V);
-        co[2][n] = c[n] / (6.0 * V);
-        co[3][n] = d[n] / (6.0 * V);
-    }
+    or_ref_coeff(a, b, c, d, V, co);
     static __thread cplx F[MAXDOF][3][NQ], CF[MAXDOF][3][NQ], MsCF[MAXDOF][3][NQ],
         MmF[MAXDOF][3][NQ];
-    double g[4][3];
-    for (int n = 0; n < 4; ++n) {
-        g[n][0] = co[1][n];
-        g[n][1] = co[2][n];
-        g[n][2] = co[3][n];
+    int ijk[MAXDOF][3];
+    for (int i = 0; i < ndof; ++i) {
+        if (typ[i] == 0) {
+            ijk[i][0] = (int)lem[0][idx[i]];
+            ijk[i][1] = (int)lem[1][idx[i]];
+            ijk[i][2] = 0;
+        } else {
+            ijk[i][0] = (int)ltm[0][idx[i]];
+            ijk[i][1] = (int)ltm[1][idx[i]];
+            ijk[i][2] = (int)ltm[2][idx[i]];
+        }
     }
     for (int q = 0; q < NQ; ++q) {
-        double px = vx[0][0] * DPTS[1][q] + vx[0][1] * DPTS[2][q] + vx[0][2] * DPTS[3][q] +
-                    vx[0][3] * DPTS[4][q];
-        double py = vx[1][0] * DPTS[1][q] + vx[1][1] * DPTS[2][q] + vx[1][2] * DPTS[3][q] +
-                    vx[1][3] * DPTS[4][q];
-        double pz = vx[2][0] * DPTS[1][q] + vx[2][1] * DPTS[2][q] + vx[2][2] * DPTS[3][q] +
-                    vx[2][3] * DPTS[4][q];
-        double l[4];
-        for (int n = 0; n < 4; ++n) l[n] = co[0][n] + co[1][n] * px + co[2][n] * py + co[3][n] * pz;
-        for (int i = 0; i < ndof; ++i) {
-            int i1, j1, k1;
-            if (typ[i] == 0) {
-                i1 = (int)lem[0][idx[i]];
-                j1 = (int)lem[1][idx[i]];
-                k1 = 0;
-            } else {
-                i1 = (int)ltm[0][idx[i]];
-                j1 = (int)ltm[1][idx[i]];
-                k1 = (int)ltm[2][idx[i]];
-            }
-            double f[3], cf[3];
-            int st = or_basis((int)codes[i], i1, j1, k1, l, g, f, cf);
-            if (st) return st;
+        double f[MAXDOF][3], cf[MAXDOF][3];
+        int st = or_ref_eval_point(co, vx, q, codes, (const int(*)[3])ijk, ndof, f, cf);
+        if (st) return OR_EBADCODE;
+        for (int i = 0; i < ndof; ++i)
             for (int r = 0; r < 3; ++r) {
-                F[i][r][q] = f[r];
-                CF[i][r][q] = cf[r];
+                F[i][r][q] = f[i][r];
+                CF[i][r][q] = cf[i][r];
             }
-        }
     }
     for (int i = 0; i < ndof; ++i) {
         for (int q = 0; q < NQ; ++q) {
@@ -325,7 +247,7 @@ int ned2_oracle_build(const double *nodes, int64_t Nn, const int64_t *tets, int6
             urt[k] = ur[2 * (k * T + t)] + I * ur[2 * (k * T + t) + 1];
             ert[k] = er[2 * (k * T + t)] + I * er[2 * (k * T + t) + 1];
         }
-        or_matinv(urt, Ms);
+        or_ref_matinv(urt, Ms);
         cplx Ke[MAXDOF * MAXDOF], Me[MAXDOF * MAXDOF];
         int st = or_tet_stiff_mass(vx, lem, ltm, Ms, ert, dofcodes, typ, idx, ndof, Ke, Me);
         if (st) {
@@ -465,7 +387,7 @@ int ned2_oracle_interp(const double *nodes, int64_t Nn, const int64_t *tets, int
             for (int r = 0; r < 3; ++r) ltm[r][f] = l3[r];
         }
         double a[4], b[4], c[4], d[4], V;
-        or_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
+        or_ref_tet_coefficients(vx[0], vx[1], vx[2], a, b, c, d, &V);
         double g[4][3], l[4];
         const double px = coords[j] - o[0], py = coords[n + j] - o[1], pz = coords[2 * n + j] - o[2];
         for (int k = 0; k < 4; ++k) {

@@ -2,11 +2,13 @@
 curlcurl.py:222-456). Indices bit-exact, FP64 values <= 1e-12 relative Frobenius,
 mixed precision <= 1e-6 against FP64.
 
-Delaunay meshes contain sliver tets on which the reference's absolute-coordinate
-cofactor sums (mth/optimized.py:142-167) lose up to ~1e-10 relative accuracy against
-the exact integral; those meshes are compared with the oracle evaluated in each tet's
-local frame (same algorithm, translated coordinates), which the closed-form tests pin
-to the exact integrals."""
+The FP64 kernel computes the exact element integrals. On Delaunay meshes the reference's
+absolute-coordinate cofactor sums (curlcurl.py:125-213) cancel on thin tets and its
+values sit 1.1e-11 to 3.6e-7 (relative Frobenius, depending on the mesh) from the exact
+integrals, so those meshes are compared with the oracle evaluated in each tet's local
+frame (same algorithm, translated coordinates), which the closed-form tests pin to the
+exact integrals. The distance to the reference's own arithmetic, and the mode that
+reproduces it, are tested in test_gpu_parity_reference.py."""
 import numpy as np
 import pytest
 

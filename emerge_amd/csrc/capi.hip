@@ -10,6 +10,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -792,10 +795,37 @@ int em_ned2_pec_mask(em_ned2_plan_t plan, const int64_t* conductor_tets, int64_t
     return EM_OK;
 }
 
-int em_material_prep(const void* er, const void* tand, const void* cond, int64_t n_tets,
-                     double omega, double pec_limit, double surf_limit, void* er_out,
-                     int64_t* conductor_tets, int64_t* n_conductor,
-                     int32_t* frequency_dependent, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Device scratch of em_material_prep, per device: grown when a call needs more, reused by
+// every later call, held until the process ends (T/64 * 8 + 9T/256 bytes for the largest
+// T seen). The lock serialises the calls on one device; each call blocks until its work
+// is done, so the buffer is free again when the lock is released.
+struct PrepScratchCache {
+    std::mutex mu;
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_prep_caches_mu;
+std::map<int, std::unique_ptr<PrepScratchCache>> g_prep_caches;
+
+PrepScratchCache& prep_cache(int device) {
+    std::lock_guard<std::mutex> lk(g_prep_caches_mu);
+    auto& c = g_prep_caches[device];
+    if (!c) c.reset(new PrepScratchCache());
+    return *c;
+}
+
+}  // namespace
+
+namespace em {
+
+// Shared argument checks of the material preparation entry points.
+int material_prep_check_args(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                             double omega, const void* er_out, const int64_t* conductor_tets,
+                             const int64_t* n_conductor, const int32_t* frequency_dependent) {
     if (n_tets < 0 || !n_conductor || !frequency_dependent ||
         (n_tets > 0 && (!er || !tand || !cond || !er_out || !conductor_tets))) {
         set_error("material_prep: bad argument");
@@ -805,15 +835,44 @@ int em_material_prep(const void* er, const void* tand, const void* cond, int64_t
         set_error("material_prep: omega must be positive");
         return EM_EINVAL;
     }
-    if (9 * n_tets >= (1ll << 40)) {
+    if (9 * n_tets > (1ll << 40) - 256) {  // one thread per component, 2^32 workgroups
         set_error("material_prep: too many tets");
         return EM_EINVAL;
+    }
+    return EM_OK;
+}
+
+}  // namespace em
+
+extern "C" {
+
+int em_material_prep(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                     double omega, double pec_limit, double surf_limit, void* er_out,
+                     int64_t* conductor_tets, int64_t* n_conductor,
+                     int32_t* frequency_dependent, void* stream) {
+    if (int st = em::material_prep_check_args(er, tand, cond, n_tets, omega, er_out,
+                                              conductor_tets, n_conductor, frequency_dependent))
+        return st;
+    *n_conductor = 0;
+    *frequency_dependent = 0;
+    if (n_tets == 0) return EM_OK;
+    int dev = 0;
+    EM_HIP(hipGetDevice(&dev));
+    PrepScratchCache& c = prep_cache(dev);
+    std::lock_guard<std::mutex> lk(c.mu);
+    const size_t need = em::material_prep_scratch_bytes(n_tets);
+    if (need > c.bytes) {
+        (void)hipFree(c.p);
+        c.p = nullptr;
+        c.bytes = 0;
+        EM_HIP(hipMalloc(&c.p, need));
+        c.bytes = need;
     }
     long long nc = 0;
     bool fd = false;
     EM_HIP(em::material_prep((const double2*)er, (const double2*)tand, (const double2*)cond,
                              n_tets, omega, pec_limit, surf_limit, (double2*)er_out,
-                             (long long*)conductor_tets, nc, fd, (hipStream_t)stream));
+                             (long long*)conductor_tets, c.p, nc, fd, (hipStream_t)stream));
     *n_conductor = nc;
     *frequency_dependent = fd ? 1 : 0;
     return EM_OK;

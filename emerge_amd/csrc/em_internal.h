@@ -142,14 +142,22 @@ hipError_t pec_mark(const int* t2f, long long n_tets, const long long* cond, lon
                     const long long* tri_to_field, long long n_tris, const long long* pec_tris,
                     long long n_pec, long long n_field, unsigned char* keep, long long& bad,
                     hipStream_t s);
-// new_index[d] = position of d among the kept DOFs or -1; solve_ids = kept DOFs ascending.
 // er' = er(1 - j tand) - j cond/(omega eps0) over (3,3,T) complex128 tensors (out may be
 // er), the ascending ids of the conductor tets (cond[0,0] above either limit) and whether
-// some component lies strictly between 0 and pec_limit. Blocks until done.
+// some component lies strictly between 0 and pec_limit (materials.hip). Device scratch of
+// material_prep_scratch_bytes(T) bytes (256-byte aligned; every byte the call reads it
+// writes first, so it needs no clearing). Blocks until done.
+size_t material_prep_scratch_bytes(long long n_tets);
+// Argument checks shared by the C-ABI material preparation entry points (capi.hip):
+// EM_OK or EM_EINVAL with the message set.
+int material_prep_check_args(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                             double omega, const void* er_out, const int64_t* conductor_tets,
+                             const int64_t* n_conductor, const int32_t* frequency_dependent);
 hipError_t material_prep(const double2* er, const double2* tand, const double2* cond,
                          long long n_tets, double omega, double pec_limit, double surf_limit,
-                         double2* out, long long* conductor_tets, long long& n_conductor,
-                         bool& frequency_dependent, hipStream_t s);
+                         double2* out, long long* conductor_tets, void* scratch,
+                         long long& n_conductor, bool& frequency_dependent, hipStream_t s);
+// new_index[d] = position of d among the kept DOFs or -1; solve_ids = kept DOFs ascending.
 hipError_t dof_compact(long long n, const unsigned char* keep, long long* new_index,
                        long long* solve_ids, long long& n_solve, hipStream_t s);
 hipError_t csc_select_count(long long n, const long long* indptr, const long long* indices,

@@ -800,8 +800,8 @@ int em_ned2_pec_mask(em_ned2_plan_t plan, const int64_t* conductor_tets, int64_t
 namespace {
 
 // Device scratch of em_material_prep, per device: grown when a call needs more, reused by
-// every later call, held until the process ends (T/64 * 8 + 9T/256 bytes for the largest
-// T seen). The lock serialises the calls on one device; each call blocks until its work
+// every later call, held until the process ends (about T/64 * 8 + 9T/256 + T/32 bytes for
+// the largest T seen). The lock serialises the calls on one device; each call blocks until its work
 // is done, so the buffer is free again when the lock is released.
 struct PrepScratchCache {
     std::mutex mu;

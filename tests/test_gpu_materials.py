@@ -32,7 +32,7 @@ def _close(got, ref, er, tand, cond):
     return bool(np.all(np.abs(got - ref) <= 4 * 2.0 ** -52 * scale))
 
 
-@pytest.mark.parametrize("T", [0, 1, 7, 20000])
+@pytest.mark.parametrize("T", [0, 1, 7, 20000, 700_001])
 def test_material_prep_matches_oracle(T):
     import torch
     from emerge_amd.materials import prepare_materials

@@ -100,6 +100,9 @@ _SIGS = {
     "em_material_prep": (ctypes.c_int, [P, P, P, I64, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, P, P, ctypes.POINTER(I64),
                                         ctypes.POINTER(I32), P]),
+    "em_material_prep_noalloc": (ctypes.c_int, [P, P, P, I64, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, P, P, ctypes.POINTER(I64),
+                                                ctypes.POINTER(I32), P]),
     "em_csc_select_count": (ctypes.c_int, [I64, P, P, P, I32, I32, P, I64, P,
                                            ctypes.POINTER(I64), P]),
     "em_csc_select_fill": (ctypes.c_int, [I64, P, P, P, I32, I32, P, P, P, P, P]),

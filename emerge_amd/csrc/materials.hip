@@ -27,6 +27,8 @@
 // is about T/64 * 8 + 9T/256 + T/32 bytes.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
 
 #include "em_internal.h"
 
@@ -245,3 +247,101 @@ hipError_t material_prep(const double2* er, const double2* tand, const double2* 
 }
 
 }  // namespace em
+
+// ---------------------------------------------------------------------------------------
+// em_material_prep_noalloc: em_material_prep for callers that cannot allocate device
+// memory. The scratch is `scratch`, a file-scope device array of kNoAllocScratchBytes
+// (4 MiB: material_prep_scratch_bytes(2^24) = 3,212,288 bytes), one copy per GPU in each
+// process, held until the process ends. On the host the array's name stands for no device
+// address: its address on the current device is obtained once per device with
+// hipGetSymbolAddress and only that pointer is handed to the kernels. Every byte a call
+// reads from it, the call writes first, so nothing is cleared between calls. Calls from
+// the threads of one process take a per-device lock; calls that overlap on one GPU
+// without it (the array is shared) would race.
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+constexpr long long kNoAllocMaxTets = 1ll << 24;
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct NoAllocSlot {
+    std::mutex call;       // one call at a time on this device
+    void* ptr = nullptr;   // address of `scratch` on this device
+    bool resolved = false;
+};
+NoAllocSlot g_noalloc[kNoAllocMaxDevices];
+std::mutex g_noalloc_resolve;
+
+// The device address of `scratch` on device `dev` (the current device), resolved on the
+// first call on that device.
+hipError_t noalloc_scratch(int dev, void** ptr) {
+    std::lock_guard<std::mutex> lk(g_noalloc_resolve);
+    NoAllocSlot& s = g_noalloc[dev];
+    if (!s.resolved) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        s.ptr = p;
+        s.resolved = true;
+    }
+    *ptr = s.ptr;
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int em_material_prep_noalloc(const void* er, const void* tand, const void* cond,
+                                        int64_t n_tets, double omega, double pec_limit,
+                                        double surf_limit, void* er_out, int64_t* conductor_tets,
+                                        int64_t* n_conductor, int32_t* frequency_dependent,
+                                        void* stream) {
+    if (int st = em::material_prep_check_args(er, tand, cond, n_tets, omega, er_out,
+                                              conductor_tets, n_conductor, frequency_dependent))
+        return st;
+    if (n_tets > kNoAllocMaxTets) {
+        em::set_error("material_prep_noalloc: " + std::to_string(n_tets) +
+                      " tets exceed the fixed scratch (at most 16,777,216 tets); use "
+                      "em_material_prep");
+        return EM_EINVAL;
+    }
+    *n_conductor = 0;
+    *frequency_dependent = 0;
+    if (n_tets == 0) return EM_OK;
+    if (em::material_prep_scratch_bytes(n_tets) > kNoAllocScratchBytes) {
+        em::set_error("material_prep_noalloc: scratch too small");
+        return EM_EINVAL;
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        em::set_error(std::string("material_prep_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        em::set_error("material_prep_noalloc: device index beyond 63");
+        return EM_EUNSUPPORTED;
+    }
+    void* ptr = nullptr;
+    if ((e = noalloc_scratch(dev, &ptr)) != hipSuccess) {
+        em::set_error(std::string("material_prep_noalloc: hipGetSymbolAddress: ") +
+                      hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc[dev].call);
+    long long nc = 0;
+    bool fd = false;
+    e = em::material_prep((const double2*)er, (const double2*)tand, (const double2*)cond, n_tets,
+                          omega, pec_limit, surf_limit, (double2*)er_out,
+                          (long long*)conductor_tets, ptr, nc, fd, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        em::set_error(std::string("material_prep_noalloc: ") + hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    *n_conductor = nc;
+    *frequency_dependent = fd ? 1 : 0;
+    return EM_OK;
+}

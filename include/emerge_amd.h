@@ -110,6 +110,20 @@ int em_material_prep(const void* er, const void* tand, const void* cond, int64_t
                      int32_t* frequency_dependent, void* stream);
 
 /*
+ * em_material_prep without any device allocation, for callers that cannot allocate (code
+ * between allocation-free sections of a serving loop, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments, results and blocking behaviour as
+ * em_material_prep; its scratch is a fixed 4 MiB device array per GPU, held until the
+ * process ends, so n_tets is limited to 16,777,216 (EM_EINVAL beyond). Every call on a GPU
+ * shares that array and leaves its contents behind; calls that overlapped on one GPU would
+ * race on it, so the library serialises the calls of one process per device.
+ */
+int em_material_prep_noalloc(const void* er, const void* tand, const void* cond, int64_t n_tets,
+                             double omega, double pec_limit, double surf_limit, void* er_out,
+                             int64_t* conductor_tets, int64_t* n_conductor,
+                             int32_t* frequency_dependent, void* stream);
+
+/*
  * Create an assembly plan: copies the mesh tables to device `device`, validates
  * them (ids in range, edge/face tables equal to the sorted-tuple tables of the tets as
  * mesh3d.py:707-774 builds them, no zero-volume tet) and keeps them resident.

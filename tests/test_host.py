@@ -104,6 +104,22 @@ def test_material_prep_rejects_bad_arguments_without_gpu():
     assert st == _lib.EM_EINVAL and n.value == -5 and fd.value == -5
 
 
+def test_material_prep_noalloc_refuses_beyond_its_scratch():
+    """em_material_prep_noalloc's fixed scratch covers 16,777,216 tets; one more is refused
+    with EM_EINVAL before any device call, as are the arguments em_material_prep refuses."""
+    L = _lib.lib()
+    n = ctypes.c_int64(-5)
+    fd = ctypes.c_int32(-5)
+    fake = ctypes.c_void_p(4096)  # never dereferenced: refused first
+    st = L.em_material_prep_noalloc(fake, fake, fake, (1 << 24) + 1, 1.0, 1e8, 1e4, fake, fake,
+                                    ctypes.byref(n), ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL and b"16,777,216" in L.em_last_error()
+    st = L.em_material_prep_noalloc(fake, fake, fake, 10, 0.0, 1e8, 1e4, fake, fake,
+                                    ctypes.byref(n), ctypes.byref(fd), None)
+    assert st == _lib.EM_EINVAL and b"omega" in L.em_last_error()
+    assert n.value == -5 and fd.value == -5
+
+
 def test_status_to_exception():
     with pytest.raises(_lib.UnsupportedError):
         _lib.check(_lib.EM_EUNSUPPORTED)

@@ -121,6 +121,53 @@ def cpu_baseline(field, er_ref, ur_ref, seconds: float, chunk: int, nthreads: in
     return done / el, nthreads, done, el
 
 
+def csc_per_frequency(plan, eps, mu, k0, steps: int, warmup: int) -> dict:
+    """The per-frequency CSC matrix of the sweep (assembler.py:494, once per frequency in
+    the reference): with K and M cached, their CSC sums are formed once and each frequency
+    is one streaming pass A = K_csc - k0^2 M_csc (em_csc_freq_combine: 2 c bytes read and c
+    written per CSC entry). Beside it, the fused column gather over the COO values that the
+    reference's order implies, to_csc(E - k0^2 B). complex64 (mixed) and complex128 (the
+    reference's dtype); scalar eps_r and diagonal mu_r give symmetric blocks (mirrored
+    gather)."""
+    import torch
+    from emerge_amd.assembly import CscMap, FreqCsc, csc_freq_combine
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cm = CscMap(plan)
+    torch.cuda.synchronize()
+    res = {"nnz": cm.nnz, "map_ms_one_time": (time.perf_counter() - t0) * 1e3,
+           "kernel": "csc_freq_kernel (em_csc_freq_combine)"}
+    for prec, c in (("mixed", 8), ("fp64", 16)):
+        K, M = plan.assemble_km(eps, mu, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fc = FreqCsc(cm, K, M, block_symmetric=True)
+        torch.cuda.synchronize()
+        sums_ms = (time.perf_counter() - t0) * 1e3
+        out = torch.empty(cm.nnz, dtype=fc.K.dtype, device="cuda")
+        ms, _ = time_launches(lambda: csc_freq_combine(fc.K, fc.M, k0, out=out), steps, warmup)
+        ref = cm.to_csc_combine(K, M, k0, block_symmetric=True)
+        torch.cuda.synchronize()
+        diff = float(torch.linalg.norm((out - ref).to(torch.complex128)) /
+                     torch.linalg.norm(ref.to(torch.complex128)))
+        del ref
+        ms_gather, _ = time_launches(
+            lambda: cm.to_csc_combine(K, M, k0, out=out, block_symmetric=True),
+            max(2, steps // 2), 1)
+        b = 3 * cm.nnz * c
+        res[prec] = {"ms_per_frequency": ms, "bytes_per_frequency": b,
+                     "gb_per_s": b / (ms * 1e-3) / 1e9,
+                     "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "ms_one_time_csc_sums_of_K_and_M": sums_ms,
+                     "ms_per_frequency_fused_coo_gather": ms_gather,
+                     "rel_diff_vs_fused_gather": diff}
+        log(f"csc per frequency ({prec}): {ms:.3f} ms combine, {ms_gather:.3f} ms fused gather")
+        del K, M, fc, out
+        torch.cuda.empty_cache()
+    cm.close()
+    return res
+
+
 def _mats(T, rng, mu_kind="diag"):
     import numpy as np
     import torch
@@ -156,11 +203,41 @@ def suite_sweep(args) -> dict:
     del sg, ref
     torch.cuda.empty_cache()
     out_bytes = 64 * T * 400 * 8
-    return {"n_tets": T, "n_freq": 64, "output": "complex64 COO per frequency",
-            "ms_per_sweep_graph": ms_graph, "ms_per_sweep_eager": ms_eager,
-            "tet_freqs_per_s": 64 * T / (ms_graph * 1e-3),
-            "write_gb_per_s": out_bytes / (ms_graph * 1e-3) / 1e9,
-            "rel_diff_replay_vs_single_launch": diff}
+    res = {"n_tets": T, "n_freq": 64, "output": "complex64 COO per frequency",
+           "ms_per_sweep_graph": ms_graph, "ms_per_sweep_eager": ms_eager,
+           "tet_freqs_per_s": 64 * T / (ms_graph * 1e-3),
+           "write_gb_per_s": out_bytes / (ms_graph * 1e-3) / 1e9,
+           "rel_diff_replay_vs_single_launch": diff}
+    # the reference's product: one CSC matrix per frequency (assembler.py:494), K and M shared
+    from emerge_amd.assembly import CscMap, FreqCsc, csc_freq_combine
+    from emerge_amd.graph import CscSweepGraph
+    cm = CscMap(plan)
+    cg = CscSweepGraph(plan, cm, one, one, k0s, "mixed")
+    log(f"csc sweep: nnz {cm.nnz}, output {cg.out.numel() * 8 / 1e9:.1f} GB")
+    ms_csc_graph, _ = time_launches(lambda: cg.replay(), 3, 1)
+    fc = FreqCsc(cm, cg.K, cg.M, block_symmetric=True)
+    ms_combine, _ = time_launches(lambda: csc_freq_combine(fc.K, fc.M, k0s, out=cg.out), 3, 1)
+    ref = cm.to_csc_combine(cg.K, cg.M, float(k0s[17]), block_symmetric=True)
+    cg.replay()
+    torch.cuda.synchronize()
+    dcsc = float(torch.linalg.norm((cg.out[17] - ref).to(torch.complex128)) /
+                 torch.linalg.norm(ref.to(torch.complex128)))
+    # algorithmic bytes of the whole CSC sweep: K and M COO blocks written (2 x 400 x 8 per
+    # tet) and read by the two column gathers (+ 2 B row position per contributor), the CSC
+    # sums written and read once (2 x 2 x nnz x 8), 64 A_f written (64 x nnz x 8)
+    b_sweep = 2 * plan.nnz_coo * 8 + 2 * plan.nnz_coo * 10 + 4 * cm.nnz * 8 + 64 * cm.nnz * 8
+    b_combine = (2 + 64) * cm.nnz * 8
+    res["csc"] = {"nnz": cm.nnz, "output": "complex64 CSC values per frequency",
+                  "ms_per_sweep_graph": ms_csc_graph,
+                  "ms_combine_64_frequencies": ms_combine,
+                  "combine_gb_per_s": b_combine / (ms_combine * 1e-3) / 1e9,
+                  "combine_roofline_frac": b_combine / (ms_combine * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "sweep_gb_per_s": b_sweep / (ms_csc_graph * 1e-3) / 1e9,
+                  "tet_freqs_per_s": 64 * T / (ms_csc_graph * 1e-3),
+                  "rel_diff_vs_fused_gather_f17": dcsc}
+    del cg, fc, ref, cm
+    torch.cuda.empty_cache()
+    return res
 
 
 def suite_batch(args) -> dict:
@@ -611,6 +688,8 @@ def main() -> int:
     ap.add_argument("--cpu-chunk", type=int, default=50_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-csc", action="store_true",
+                    help="skip the per-frequency CSC line (CSC map, K/M sums, A_f combine)")
     ap.add_argument("--suite", action="store_true",
                     help="also measure the frequency sweep (500k tets x 64 f), the uneven "
                          "32-mesh batch and the backward on 1M tets")
@@ -689,6 +768,10 @@ def main() -> int:
         del src, dst
         log("copy calibration: 2 GiB device copy done")
 
+    csc = None
+    if not args.no_csc:
+        csc = csc_per_frequency(plan, eps, mu, k0, args.steps, args.warmup)
+
     b32 = bytes_per_tet(8, Nn / T)
     achieved = T * b32 / (ms32 * 1e-3) / 1e9
     workload_key = f"delaunay_p{args.points}_s{args.seed}_mixed"
@@ -747,6 +830,7 @@ def main() -> int:
                      "bytes_per_tet": b32, "kernel": "ned2_assemble_kernel<float, MODE_A>"},
         "fp64": res64,
         "tuning_nb": tuning,
+        "csc_per_frequency": csc,
         "cpu_baseline": cpu,
     }
     if args.suite or args.target_points:

@@ -9,7 +9,9 @@ converts K and M separately instead of combining them):
    dependence and the conductor tets (:439-469) -- ``em_material_prep``;
 3. the element matrices without the conductor tets (:479-485), cached as the reference's
    ``cached_matrices`` and reused while nothing is frequency dependent (:471-478);
-4. ``cached_cscmap.to_csc(Evec - Bvec * K0**2)`` (:494) -- the fused CSC combine.
+4. ``cached_cscmap.to_csc(Evec - Bvec * K0**2)`` (:494) -- the fused CSC combine, or,
+   while the element matrices are reused, ``K_csc - K0**2 M_csc`` from CSC sums of K and M
+   kept with them (``FreqCsc``: the same matrix up to the order of each entry's sum).
 
 Everything stays on the GPU; the result is the CSC value array of ``A`` with the map's
 ``indptr``/``indices``. The cached matrices and CSC map belong to the conductor set they
@@ -26,7 +28,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .assembly import CscMap, Material
+from .assembly import CscMap, FreqCsc, Material
 from .curlcurl import plan_for
 from .materials import PECLIM, SURFIMPLIM, prepare_materials
 
@@ -83,6 +85,7 @@ class VolumeAssembler:
         self.surfimplim = surfimplim
         self.cached_matrices = None  # (plan, K, M, blocks symmetric)
         self._map = None  # (plan, CscMap)
+        self._freq = None  # (cached K, FreqCsc of the cached K and M)
         self.assemblies = 0
 
     def assemble_freq_matrix(self, er, ur, tand, cond, frequency: float,
@@ -106,7 +109,14 @@ class VolumeAssembler:
             self.assemblies += 1
             self.cached_matrices = (plan, K, M, sym)
         cm = self._cscmap(plan)
-        data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
+        if cache_matrices and not fd:
+            # while K and M are reused, so are their CSC sums: each frequency is then one
+            # streaming pass A = K_csc - K0^2 M_csc instead of a gather over the COO values
+            if self._freq is None or self._freq[0] is not K:
+                self._freq = (K, FreqCsc(cm, K, M, block_symmetric=sym))
+            data = self._freq[1].A(K0)
+        else:
+            data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
         return FreqMatrix(data, cm, conductor_tets, fd, K0)
 
     def assemble_eig_matrices(self, er, ur, tand, cond, frequency: float):

@@ -438,6 +438,60 @@ class CscMap:
         return out
 
 
+def csc_freq_combine(K: torch.Tensor, M: torch.Tensor, k0, out=None,
+                     stream=None) -> torch.Tensor:
+    """``A = K - k0**2 * M`` for CSC value arrays ``K``, ``M`` of one pattern
+    (``em_csc_freq_combine``): ``k0`` a float gives ``[nnz]``, a sequence ``[len(k0), nnz]``
+    in one pass over ``K`` and ``M``. complex128 or complex64 (FP64 arithmetic)."""
+    if K.dtype != M.dtype or K.numel() != M.numel() or \
+            K.dtype not in (torch.complex128, torch.complex64):
+        raise ValueError("K and M must be complex CSC value arrays of one pattern")
+    K, M = K.contiguous(), M.contiguous()
+    prec = PREC_FP64 if K.dtype == torch.complex128 else PREC_MIXED
+    scalar = np.isscalar(k0)
+    k0a = np.ascontiguousarray(np.atleast_1d(np.asarray(k0, dtype=np.float64)))
+    nf, n = int(k0a.shape[0]), K.numel()
+    if out is None:
+        out = torch.empty((nf, n), dtype=K.dtype, device=K.device)
+    else:
+        _check_out(out, nf * n, K.dtype)
+    stride = n if out.dim() == 1 else out.stride(0)
+    check(_lib.lib().em_csc_freq_combine(n, ctypes.c_void_p(K.data_ptr()),
+                                         ctypes.c_void_p(M.data_ptr()),
+                                         k0a.ctypes.data_as(ctypes.c_void_p), nf,
+                                         ctypes.c_void_p(out.data_ptr()), stride, prec,
+                                         _stream_ptr(stream)))
+    if scalar and out.dim() == 2:
+        return out[0]
+    return out
+
+
+class FreqCsc:
+    """CSC values of a plan's K and M, summed once, and ``A(k0) = K - k0**2 M`` per
+    frequency as one streaming pass (``em_csc_freq_combine``).
+
+    The reference converts ``Evec - Bvec * K0**2`` to CSC at every frequency
+    (assembler.py:471-494, called per frequency from microwave_3d.py:1464-1470); while the
+    element matrices are cached the sums of K and M are too. Each entry is then summed as
+    ``sum(E) - k0**2 sum(B)`` instead of ``sum(E - k0**2 B)``: the same matrix up to
+    rounding. ``block_symmetric``: the blocks of K and M are symmetric (scalar or diagonal
+    materials), which selects the mirrored column gather.
+    """
+
+    def __init__(self, cscmap: CscMap, K: torch.Tensor, M: torch.Tensor,
+                 block_symmetric: bool = False, stream=None):
+        self.map = cscmap
+        self.K = cscmap.to_csc(K, block_symmetric=block_symmetric, stream=stream)
+        self.M = cscmap.to_csc(M, block_symmetric=block_symmetric, stream=stream)
+
+    @property
+    def nnz(self) -> int:
+        return self.map.nnz
+
+    def A(self, k0, out=None, stream=None) -> torch.Tensor:
+        return csc_freq_combine(self.K, self.M, k0, out=out, stream=stream)
+
+
 def _check_out(t: torch.Tensor, numel: int, dt) -> None:
     if t.dtype != dt or not t.is_cuda or t.numel() != numel:
         raise ValueError(f"output must be a {dt} CUDA tensor with {numel} elements")

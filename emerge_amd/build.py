@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemerge_amd.so")
-SOURCES = ["capi.hip", "ned2_assemble.hip", "ned2_ref.hip", "ned2_backward.hip", "csc_map.hip", "dof_select.hip", "robin.hip",
+SOURCES = ["capi.hip", "ned2_assemble.hip", "ned2_ref.hip", "ned2_backward.hip", "csc_map.hip", "csc_freq.hip", "dof_select.hip", "robin.hip",
            "interp.hip", "heat.hip", "materials.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

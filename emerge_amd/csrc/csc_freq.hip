@@ -1,0 +1,100 @@
+// Per-frequency CSC values A_f = K - k0_f^2 M from the CSC values of K and M, which share
+// one pattern (the map of the plan).
+//
+// The reference combines the cached element matrices before its COO -> CSC step,
+// to_csc(Evec - Bvec * K0**2) (physics/microwave/assembly/assembler.py:471-494), so every
+// frequency of a sweep (microwave_3d.py:1464-1470) pays a full gather over the COO values.
+// K and M do not change with the frequency: summed once per mesh by the column gather,
+// each frequency is then one streaming pass over the CSC values. The matrix is the same;
+// each entry is summed in another order (sum(E) - k0^2 sum(B) instead of sum(E - k0^2 B)),
+// which differs by rounding only.
+//
+// HBM-bound, no reuse: per launch K and M are read once (2 c bytes per entry) and n_k0
+// value arrays written (n_k0 c bytes per entry), c = 16 (complex128) or 8 (complex64). One
+// thread per 16-byte vector: one complex128 entry, or two complex64 entries. The sums are
+// FP64 for both (complex64 inputs are widened, the result rounded once). Stores are
+// non-temporal: an output is not read again by this launch.
+#include <hip/hip_runtime.h>
+
+#include "em_internal.h"
+
+namespace em {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void csc_freq_kernel_c128(const double2* __restrict__ K,
+                                                            const double2* __restrict__ M,
+                                                            long long n, K0Pack k0p, int nf,
+                                                            double2* __restrict__ out,
+                                                            long long stride) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double2 k = K[i], m = M[i];
+    for (int f = 0; f < nf; ++f) {
+        const double k2 = k0p.v[f];
+        const f64x2 a = {fma(-k2, m.x, k.x), fma(-k2, m.y, k.y)};
+        __builtin_nontemporal_store(a, reinterpret_cast<f64x2*>(out + (long long)f * stride + i));
+    }
+}
+
+// complex64: thread i owns entries 2i and 2i + 1 (one 16-byte load of K, of M and store
+// per output); an odd n leaves the last entry to the thread of pair n / 2 alone.
+__global__ __launch_bounds__(256) void csc_freq_kernel_c64(const float2* __restrict__ K,
+                                                           const float2* __restrict__ M,
+                                                           long long n, K0Pack k0p, int nf,
+                                                           float2* __restrict__ out,
+                                                           long long stride) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long e = 2 * p;
+    if (e >= n) return;
+    if (e + 1 < n) {
+        const f32x4 k = *reinterpret_cast<const f32x4*>(K + e);
+        const f32x4 m = *reinterpret_cast<const f32x4*>(M + e);
+        for (int f = 0; f < nf; ++f) {
+            const double k2 = k0p.v[f];
+            const f32x4 a = {(float)fma(-k2, (double)m.x, (double)k.x),
+                             (float)fma(-k2, (double)m.y, (double)k.y),
+                             (float)fma(-k2, (double)m.z, (double)k.z),
+                             (float)fma(-k2, (double)m.w, (double)k.w)};
+            __builtin_nontemporal_store(a, reinterpret_cast<f32x4*>(out + (long long)f * stride + e));
+        }
+    } else {
+        const float2 k = K[e], m = M[e];
+        for (int f = 0; f < nf; ++f) {
+            const double k2 = k0p.v[f];
+            out[(long long)f * stride + e] =
+                make_float2((float)fma(-k2, (double)m.x, (double)k.x),
+                            (float)fma(-k2, (double)m.y, (double)k.y));
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t csc_freq_combine(long long n, const void* K, const void* M, const double* k0sq, int nf,
+                            void* out, long long stride, int complex64, hipStream_t s) {
+    if (n == 0 || nf == 0) return hipSuccess;
+    const long long threads = complex64 ? (n + 1) / 2 : n;
+    const unsigned grid = (unsigned)((threads + 255) / 256);
+    for (int f0 = 0; f0 < nf; f0 += kMaxK0) {
+        K0Pack p;
+        const int cnt = min(kMaxK0, nf - f0);
+        for (int f = 0; f < kMaxK0; ++f) p.v[f] = f < cnt ? k0sq[f0 + f] : 0.0;
+        if (complex64)
+            hipLaunchKernelGGL(csc_freq_kernel_c64, dim3(grid), dim3(256), 0, s,
+                               (const float2*)K, (const float2*)M, n, p, cnt,
+                               (float2*)out + (long long)f0 * stride, stride);
+        else
+            hipLaunchKernelGGL(csc_freq_kernel_c128, dim3(grid), dim3(256), 0, s,
+                               (const double2*)K, (const double2*)M, n, p, cnt,
+                               (double2*)out + (long long)f0 * stride, stride);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace em

@@ -62,13 +62,64 @@ class SweepGraph:
         self.er = plan.material(er)
         self.ur = plan.material(ur)
         self.k0s = np.ascontiguousarray(np.asarray(k0s, np.float64).ravel())
-        dt = torch.complex128 if precision in ("fp64", 0) else torch.complex64
+        from .assembly import _out_dtype, _prec
+        dt = _out_dtype(_prec(precision))
         if out is None:
             out = torch.empty((self.k0s.shape[0], plan.nnz_coo), dtype=dt, device=plan.device)
         self.out = out
         self.precision = precision
         self.graph = Graph(lambda: plan.assemble_a(self.er, self.ur, self.k0s, precision,
                                                    out=self.out))
+
+    def replay(self, stream=None) -> torch.Tensor:
+        self.graph.replay(stream)
+        return self.out
+
+
+class CscSweepGraph:
+    """A frequency sweep in CSC form as one graph: the element matrices K and M of every
+    tet (``em_ned2_assemble_km``), their CSC sums (two column gathers) and every
+    ``A_f = K - k0_f**2 M`` (``em_csc_freq_combine``, one pass per 128 wavenumbers).
+
+    This is the reference's sweep (microwave_3d.py:1464-1470 calling
+    ``assemble_freq_matrix``; ``cscmap.to_csc(Evec - Bvec * K0**2)``, assembler.py:494,
+    at each frequency) with K and M shared. ``out`` is ``[F, nnz]`` CSC values (pattern:
+    ``cscmap.export("indptr", "indices")``); complex128 for ``"fp64"``, complex64 for
+    ``"mixed"``. The COO blocks and the CSC sums of K and M stay allocated with the graph.
+    """
+
+    def __init__(self, plan, cscmap, er, ur, k0s, precision="mixed", out=None,
+                 block_symmetric=None):
+        import numpy as np
+        from .assembly import _out_dtype, _prec, csc_freq_combine
+        self.plan = plan
+        self.map = cscmap
+        self.er = plan.material(er)
+        self.ur = plan.material(ur)
+        if block_symmetric is None:
+            block_symmetric = self.er.kind != _lib.EM_MAT_FULL and \
+                self.ur.kind != _lib.EM_MAT_FULL
+        self.k0s = np.ascontiguousarray(np.asarray(k0s, np.float64).ravel())
+        prec = _prec(precision)
+        if prec == _lib.EM_PREC_FP64_REF:
+            raise ValueError("the CSC sweep takes 'fp64' or 'mixed'")
+        dt = _out_dtype(prec)
+        dev = plan.device
+        self.K = torch.empty(plan.nnz_coo, dtype=dt, device=dev)
+        self.M = torch.empty(plan.nnz_coo, dtype=dt, device=dev)
+        self.K_csc = torch.empty(cscmap.nnz, dtype=dt, device=dev)
+        self.M_csc = torch.empty(cscmap.nnz, dtype=dt, device=dev)
+        if out is None:
+            out = torch.empty((self.k0s.shape[0], cscmap.nnz), dtype=dt, device=dev)
+        self.out = out
+
+        def record():
+            plan.assemble_km(self.er, self.ur, precision, out=(self.K, self.M))
+            cscmap.to_csc(self.K, out=self.K_csc, block_symmetric=block_symmetric)
+            cscmap.to_csc(self.M, out=self.M_csc, block_symmetric=block_symmetric)
+            csc_freq_combine(self.K_csc, self.M_csc, self.k0s, out=self.out)
+
+        self.graph = Graph(record)
 
     def replay(self, stream=None) -> torch.Tensor:
         self.graph.replay(stream)

@@ -108,7 +108,6 @@ int em_material_prep(const void* er, const void* tand, const void* cond, int64_t
                      double omega, double pec_limit, double surf_limit, void* er_out,
                      int64_t* conductor_tets, int64_t* n_conductor,
                      int32_t* frequency_dependent, void* stream);
-
 /*
  * em_material_prep without any device allocation, for callers that cannot allocate (code
  * between allocation-free sections of a serving loop, harnesses that forbid hipMalloc). No
@@ -252,6 +251,20 @@ int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, dou
  */
 int em_csc_to_csc_staged(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                          void* work, void* data_csc, int32_t precision, void* stream);
+
+/*
+ * A_f = K - k0[f]^2 M for CSC values K and M of one pattern (n entries each, e.g. the
+ * em_csc_to_csc results of one map): the per-frequency to_csc(Evec - Bvec * K0**2) of
+ * assembler.py:494 with K and M summed once per mesh instead of once per frequency (each
+ * entry is summed in another order: sum(E) - k0^2 sum(B); equal up to rounding). out holds
+ * n_k0 value arrays, array f starting at element f * out_stride (out_stride >= n, even for
+ * complex64). k0 is a HOST array; its values travel as kernel arguments (one launch per 128
+ * wavenumbers), so the call allocates nothing, never synchronises and can be captured into
+ * a HIP graph. complex128 (EM_PREC_FP64) or complex64 (EM_PREC_MIXED: FP64 arithmetic,
+ * rounded once); all pointers 16-byte aligned; runs on the current device.
+ */
+int em_csc_freq_combine(int64_t n, const void* K, const void* M, const double* k0, int32_t n_k0,
+                        void* out, int64_t out_stride, int32_t precision, void* stream);
 
 /*
  * PEC elimination and the solver hand-off. All pointers are DEVICE pointers; functions

@@ -771,6 +771,7 @@ def main() -> int:
     csc = None
     if not args.no_csc:
         csc = csc_per_frequency(plan, eps, mu, k0, args.steps, args.warmup)
+        csc["mixed"]["traffic"] = load_traffic(f"delaunay_p{args.points}_s{args.seed}_csc_freq_mixed")
 
     b32 = bytes_per_tet(8, Nn / T)
     achieved = T * b32 / (ms32 * 1e-3) / 1e9

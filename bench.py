@@ -274,7 +274,7 @@ def suite_batch(args) -> dict:
     b = bytes_per_tet(8, sum(f.mesh.n_nodes for f in fields) / T, mat_bytes=32)
     return {"n_meshes": 32, "n_tets": T, "tets_per_mesh": [f.n_tets for f in fields],
             "ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
-            "gb_per_s": T * b / (ms * 1e-3) / 1e9, "tuned_nb": tune["nb"],
+            "gb_per_s": T * b / (ms * 1e-3) / 1e9, "tuned_nb": tune["nb"], "tuned_window": tune["window"],
             "rel_err_vs_fp64_per_mesh": errs, "max_rel_err": max(errs)}
 
 
@@ -736,7 +736,8 @@ def main() -> int:
 
     from emerge_amd import autotune
     a32 = torch.empty(nnz, dtype=torch.complex64, device="cuda")
-    tuning = {"mixed": autotune.tune(plan, eps, mu, k0, "mixed", out=a32[None])["nb"]}
+    r = autotune.tune(plan, eps, mu, k0, "mixed", out=a32[None])
+    tuning = {"mixed": {"tile": r["nb"], "window": r["window"]}}
     ms32, wall32 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32),
                                  args.steps, args.warmup)
     log(f"mixed: {ms32:.3f} ms/launch")
@@ -744,11 +745,13 @@ def main() -> int:
     err = None
     if not args.no_fp64:
         a64 = torch.empty(nnz, dtype=torch.complex128, device="cuda")
-        tuning["fp64"] = autotune.tune(plan, eps, mu, k0, "fp64", out=a64[None])["nb"]
+        r = autotune.tune(plan, eps, mu, k0, "fp64", out=a64[None])
+        tuning["fp64"] = {"tile": r["nb"], "window": r["window"]}
         ms64, wall64 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "fp64", out=a64),
                                      args.steps, args.warmup)
         log(f"fp64: {ms64:.3f} ms/launch")
-        plan.set_tuning(tuning["mixed"])
+        plan.set_window(tuning["mixed"]["window"])
+        plan.set_tuning(tuning["mixed"]["tile"])
         plan.assemble_a(eps, mu, k0, "mixed", out=a32)
         torch.cuda.synchronize()
         err = rel_err_chunked(a32, a64)
@@ -830,7 +833,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_tet": b32, "kernel": "ned2_assemble_kernel<float, MODE_A>"},
         "fp64": res64,
-        "tuning_nb": tuning,
+        "tuning": tuning,
         "csc_per_frequency": csc,
         "cpu_baseline": cpu,
     }

@@ -78,6 +78,8 @@ _SIGS = {
     "em_ned2_plan_destroy": (ctypes.c_int, [P]),
     "em_ned2_plan_set_tuning": (ctypes.c_int, [P, I32]),
     "em_ned2_plan_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
+    "em_ned2_plan_set_window": (ctypes.c_int, [P, I64]),
+    "em_ned2_plan_get_window": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
     "em_ned2_plan_sizes": (ctypes.c_int, [P, ctypes.POINTER(I64), ctypes.POINTER(I64),
                                           ctypes.POINTER(I64)]),
     "em_ned2_assemble_km": (ctypes.c_int, [P, ctypes.POINTER(em_material_t),

@@ -212,6 +212,15 @@ class Ned2Plan:
         check(_lib.lib().em_ned2_plan_get_tuning(self._h, ctypes.byref(v)))
         return v.value
 
+    def set_window(self, window: int) -> None:
+        """Tets per window of the tile order (rebuilds the tile lists; synchronises)."""
+        check(_lib.lib().em_ned2_plan_set_window(self._h, int(window)))
+
+    def get_window(self) -> int:
+        v = ctypes.c_int64()
+        check(_lib.lib().em_ned2_plan_get_window(self._h, ctypes.byref(v)))
+        return v.value
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.lib().em_ned2_plan_destroy(self._h)

@@ -1,12 +1,19 @@
 """Autotuner for the assembly kernels' launch shape.
 
-The tunable is the number of tets per workgroup tile (32, 64 or 128): it trades the
-latency of the per-tile geometry phase and the tail of partly filled tiles against
-occupancy, and the best value depends on the precision, the output mode (one k0 or a
-sweep), the material kinds and the mesh size.
-``tune`` times each candidate with HIP events on the launch stream, applies the fastest
-to the plan and remembers it in a JSON cache keyed by all of the above, so later plans
-of the same kind skip the timing.
+Two knobs, tuned together:
+
+* tets per workgroup tile (32, 64 or 128): the latency of the per-tile geometry phase and
+  the tail of partly filled tiles against occupancy;
+* tets per window of the tile order (512 .. 16,384): each window's tets are grouped by
+  vertex order and its tiles run on one XCD. Small windows keep the blocks written at the
+  same time close together in memory (the kernel is bound by its store stream); windows so
+  small that a vertex-order group holds a few tets leave the tiles nearly empty. The best
+  window moves with the mesh size (512 on a 2.8M-tet mesh, 1,024-4,096 on an 8M one).
+
+The best pair depends on the precision, the output mode (one k0 or a sweep), the material
+kinds and the mesh size. ``tune`` times each candidate pair with HIP events on the launch
+stream, applies the fastest to the plan and remembers it in a JSON cache keyed by all of
+the above, so later plans of the same kind skip the timing.
 
 Cache file: ``$EMERGE_AMD_TUNE_CACHE`` or ``~/.cache/emerge_amd/autotune.json``.
 """
@@ -20,6 +27,7 @@ import numpy as np
 import torch
 
 CANDIDATES = (32, 64, 128)
+WINDOWS = (512, 1024, 4096, 16384)
 
 
 def cache_path() -> str:
@@ -71,34 +79,46 @@ def _time(fn, reps: int, warmup: int) -> float:
 
 
 def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, warmup: int = 2,
-         use_cache: bool = True, candidates=CANDIDATES) -> dict:
-    """Pick the fastest tile size for ``plan.assemble_a(er, ur, k0, precision)``.
+         use_cache: bool = True, candidates=CANDIDATES, windows=WINDOWS) -> dict:
+    """Pick the fastest (window, tile) pair for ``plan.assemble_a(er, ur, k0, precision)``.
 
-    Returns ``{"nb": best, "ms": {nb: ms}, "cached": bool, "key": key}`` and leaves the
-    plan set to the best value.
+    Returns ``{"nb": tile, "window": window, "ms": {"window/tile": ms}, "cached": bool,
+    "key": key}`` and leaves the plan set to the best pair. ``windows=None`` keeps the
+    plan's window and tunes the tile only.
     """
     er_m, ur_m = plan.material(er), plan.material(ur)
     k0a = np.atleast_1d(np.asarray(k0, np.float64))
     key = tuning_key(plan, er_m, ur_m, int(k0a.shape[0]), precision)
+    if windows is None:
+        windows = (plan.get_window(),)
+    else:
+        key += "|win"
     cache = _load() if use_cache else {}
     if use_cache and key in cache:
-        plan.set_tuning(int(cache[key]["nb"]))
-        return {"nb": int(cache[key]["nb"]), "ms": cache[key].get("ms", {}), "cached": True,
-                "key": key}
+        c = cache[key]
+        plan.set_window(int(c.get("window", plan.get_window())))
+        plan.set_tuning(int(c["nb"]))
+        return {"nb": int(c["nb"]), "window": plan.get_window(), "ms": c.get("ms", {}),
+                "cached": True, "key": key}
     if out is None:
-        dt = torch.complex128 if precision == "fp64" else torch.complex64
-        out = torch.empty((k0a.shape[0], plan.nnz_coo), dtype=dt, device=plan.device)
+        from .assembly import _out_dtype, _prec
+        out = torch.empty((k0a.shape[0], plan.nnz_coo), dtype=_out_dtype(_prec(precision)),
+                          device=plan.device)
     k0arg = float(k0a[0]) if k0a.shape[0] == 1 else k0a
     target = out[0] if (k0a.shape[0] == 1 and out.dim() == 2) else out
     times = {}
-    for nb in candidates:
-        plan.set_tuning(nb)
-        times[nb] = _time(lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target),
-                          reps, warmup)
-    best = min(times, key=times.get)
+    for w in windows:
+        plan.set_window(w)
+        for nb in candidates:
+            plan.set_tuning(nb)
+            times[(w, nb)] = _time(
+                lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target), reps, warmup)
+    best_w, best = min(times, key=times.get)
+    plan.set_window(best_w)
     plan.set_tuning(best)
+    ms = {f"{w}/{nb}": v for (w, nb), v in times.items()}
     if use_cache:
         cache = _load()
-        cache[key] = {"nb": best, "ms": {str(k): v for k, v in times.items()}}
+        cache[key] = {"nb": best, "window": best_w, "ms": ms}
         _save(cache)
-    return {"nb": best, "ms": times, "cached": False, "key": key}
+    return {"nb": best, "window": best_w, "ms": ms, "cached": False, "key": key}

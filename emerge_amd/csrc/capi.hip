@@ -35,6 +35,8 @@ struct em_ned2_plan_s {
     int* t2f = nullptr;
     int* tetids = nullptr;
     int tile = 64;  // tets per workgroup tile of the assembly kernel (autotuned)
+    long long window = 1024;        // tets per code-grouping window (autotuned)
+    std::vector<unsigned char> code;  // vertex-rank code of every assembly index (host)
     int* order = nullptr;           // assembly indices grouped by vertex-rank code
     int2* tiles[3] = {nullptr, nullptr, nullptr};
     long long ntiles[3] = {0, 0, 0};
@@ -139,6 +141,57 @@ int needs_anti(const em_material_t* er, const em_material_t* ur) {
 
 }  // namespace
 
+namespace {
+
+// Tiles for the assembly kernels. The assembly range is cut into windows of `window`
+// consecutive tets (neighbours share nodes); inside each window the tets are grouped by
+// vertex-rank code (stable counting sort) and each group is cut into tiles of 32, 64 and
+// 128 tets. Workgroups are dispatched round-robin over the XCDs, so the tile list is
+// interleaved such that every tile of a window runs on the same XCD and its node/material
+// reads hit that XCD's L2: tile j*kXcds + x is the j-th tile of XCD x, which takes the
+// windows w with w % kXcds == x (shorter lists padded with empty tiles). Small windows
+// keep the blocks written at the same time close together in memory, but windows so
+// small that a code's group holds a few tets leave tiles nearly empty: on a 2.8M-tet mesh
+// 1,024-tet windows launch 4% faster than 16,384-tet ones and 512-tet windows 6%, on an
+// 8.3M-tet mesh 512-tet windows are 30% slower (tiles of ~20 tets). The autotuner picks
+// the window per mesh (em_ned2_plan_set_window); plan creation uses 1,024.
+void build_tiles(const std::vector<unsigned char>& code, long long window, std::vector<int>& horder,
+                 std::vector<std::vector<int2>>& htiles) {
+    constexpr int kXcds = 8;
+    const long long n_assy = (long long)code.size();
+    horder.assign((size_t)n_assy, 0);
+    htiles.assign(3, std::vector<int2>());
+    const long long nwin = (n_assy + window - 1) / window;
+    std::vector<std::vector<std::vector<int2>>> per_xcd(3, std::vector<std::vector<int2>>(kXcds));
+    for (long long w = 0; w < nwin; ++w) {
+        const long long w0 = w * window, w1 = std::min<long long>(n_assy, w0 + window);
+        long long cnt[65] = {0};
+        for (long long ia = w0; ia < w1; ++ia) cnt[code[ia] + 1]++;
+        for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
+        std::vector<long long> cur(cnt, cnt + 64);
+        for (long long ia = w0; ia < w1; ++ia) horder[w0 + cur[code[ia]]++] = (int)ia;
+        for (int k = 0; k < 3; ++k) {
+            const int ts = em::kTileSizes[k];
+            for (int c = 0; c < 64; ++c)
+                for (long long s0 = cnt[c]; s0 < cnt[c + 1]; s0 += ts) {
+                    const long long n = std::min<long long>(ts, cnt[c + 1] - s0);
+                    per_xcd[k][w % kXcds].push_back(
+                        make_int2((int)(w0 + s0), (int)(n | ((long long)c << 16))));
+                }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        size_t longest = 0;
+        for (int x = 0; x < kXcds; ++x) longest = std::max(longest, per_xcd[k][x].size());
+        htiles[k].assign(longest * kXcds, make_int2(0, 0));  // empty tiles: count 0
+        for (int x = 0; x < kXcds; ++x)
+            for (size_t j = 0; j < per_xcd[k][x].size(); ++j)
+                htiles[k][j * kXcds + x] = per_xcd[k][x][j];
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* em_last_error(void) { return em::g_err.c_str(); }
@@ -220,60 +273,20 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         }
     }
 
-    // Tiles for the assembly kernels. The assembly range is cut into windows of kWindow
-    // consecutive tets (neighbours share nodes); inside each window the tets are grouped
-    // by vertex-rank code (stable counting sort) and each group is cut into tiles of 32,
-    // 64 and 128 tets. Workgroups are dispatched round-robin over the XCDs, so the tile
-    // list is interleaved such that every tile of a window runs on the same XCD and its
-    // node/material reads hit that XCD's L2: tile j*kXcds + x is the j-th tile of XCD x,
-    // which takes the windows w with w % kXcds == x (shorter lists padded with empty tiles).
-    // Small windows keep the blocks written at the same time close together in memory:
-    // on a 2.8M-tet mesh 1,024-tet windows launch 4% faster than 16,384-tet ones (mixed
-    // 1.63 vs 1.69 ms) and 512-tet windows 6%; but on an 8.3M-tet mesh tiles of ~20 tets
-    // (512-tet windows, or 32-tet tiles) are 30% slower than tiles of 40-130 tets, which
-    // 1,024-tet windows keep (5.15 ms vs 5.31 ms with 16,384).
-    constexpr long long kWindow = 1024;
-    constexpr int kXcds = 8;
-    std::vector<int> horder((size_t)n_assy);
-    std::vector<std::vector<int2>> htiles(3);
-    {
-        std::vector<unsigned char> code((size_t)n_assy);
-        for (long long ia = 0; ia < n_assy; ++ia) {
-            const int4 v = htets[tetids ? hids[ia] : ia];
-            const int n[4] = {v.x, v.y, v.z, v.w};
-            int r[4];
-            for (int a = 0; a < 4; ++a)
-                r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
-            code[ia] = (unsigned char)(r[0] * 16 + r[1] * 4 + r[2]);
-        }
-        const long long nwin = (n_assy + kWindow - 1) / kWindow;
-        std::vector<std::vector<std::vector<int2>>> per_xcd(3, std::vector<std::vector<int2>>(kXcds));
-        for (long long w = 0; w < nwin; ++w) {
-            const long long w0 = w * kWindow, w1 = std::min<long long>(n_assy, w0 + kWindow);
-            long long cnt[65] = {0};
-            for (long long ia = w0; ia < w1; ++ia) cnt[code[ia] + 1]++;
-            for (int c = 0; c < 64; ++c) cnt[c + 1] += cnt[c];
-            std::vector<long long> cur(cnt, cnt + 64);
-            for (long long ia = w0; ia < w1; ++ia) horder[w0 + cur[code[ia]]++] = (int)ia;
-            for (int k = 0; k < 3; ++k) {
-                const int ts = em::kTileSizes[k];
-                for (int c = 0; c < 64; ++c)
-                    for (long long s0 = cnt[c]; s0 < cnt[c + 1]; s0 += ts) {
-                        const long long n = std::min<long long>(ts, cnt[c + 1] - s0);
-                        per_xcd[k][w % kXcds].push_back(
-                            make_int2((int)(w0 + s0), (int)(n | ((long long)c << 16))));
-                    }
-            }
-        }
-        for (int k = 0; k < 3; ++k) {
-            size_t longest = 0;
-            for (int x = 0; x < kXcds; ++x) longest = std::max(longest, per_xcd[k][x].size());
-            htiles[k].assign(longest * kXcds, make_int2(0, 0));  // empty tiles: count 0
-            for (int x = 0; x < kXcds; ++x)
-                for (size_t j = 0; j < per_xcd[k][x].size(); ++j)
-                    htiles[k][j * kXcds + x] = per_xcd[k][x][j];
-        }
+    // vertex-rank code of every assembly index (the tet's node-id order), then the tiles
+    std::vector<unsigned char> code((size_t)n_assy);
+    for (long long ia = 0; ia < n_assy; ++ia) {
+        const int4 v = htets[tetids ? hids[ia] : ia];
+        const int n[4] = {v.x, v.y, v.z, v.w};
+        int r[4];
+        for (int a = 0; a < 4; ++a)
+            r[a] = (n[0] < n[a]) + (n[1] < n[a]) + (n[2] < n[a]) + (n[3] < n[a]);
+        code[ia] = (unsigned char)(r[0] * 16 + r[1] * 4 + r[2]);
     }
+    constexpr long long kWindow = 1024;
+    std::vector<int> horder;
+    std::vector<std::vector<int2>> htiles;
+    build_tiles(code, kWindow, horder, htiles);
 
     DeviceGuard g(device);
     if (!g.ok) {
@@ -286,6 +299,7 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
     p->n_tets = n_tets;
     p->n_assy = n_assy;
     p->n_field = n_field;
+    p->window = kWindow;
     int st;
     if ((st = dmalloc(&p->xyz, hxyz.size())) || (st = dmalloc(&p->tets, htets.size())) ||
         (st = dmalloc(&p->t2f, ht2f.size())) || (st = dmalloc(&p->tetids, hids.size())) ||
@@ -363,6 +377,7 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         free_plan(p);
         return EM_EDEGENERATE;
     }
+    p->code = std::move(code);
     *plan = p;
     return EM_OK;
 }
@@ -563,6 +578,64 @@ int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t tile) {
         return EM_EINVAL;
     }
     plan->tile = tile;
+    return EM_OK;
+}
+
+int em_ned2_plan_set_window(em_ned2_plan_t plan, int64_t window) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    if (window < 32 || window > (1ll << 24)) {
+        set_error("tuning: the window must hold 32 to 16,777,216 tets");
+        return EM_EINVAL;
+    }
+    if (window == plan->window) return EM_OK;
+    std::vector<int> horder;
+    std::vector<std::vector<int2>> htiles;
+    build_tiles(plan->code, window, horder, htiles);
+    DeviceGuard g(plan->device);
+    int* order = nullptr;
+    int2* tiles[3] = {nullptr, nullptr, nullptr};
+    int st;
+    if ((st = dmalloc(&order, horder.size())) || (st = dmalloc(&tiles[0], htiles[0].size())) ||
+        (st = dmalloc(&tiles[1], htiles[1].size())) || (st = dmalloc(&tiles[2], htiles[2].size()))) {
+        (void)hipFree(order);
+        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
+        return st;
+    }
+    hipError_t e = horder.empty() ? hipSuccess
+                                  : hipMemcpy(order, horder.data(), horder.size() * sizeof(int),
+                                              hipMemcpyHostToDevice);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+        if (!htiles[k].empty())
+            e = hipMemcpy(tiles[k], htiles[k].data(), htiles[k].size() * sizeof(int2),
+                          hipMemcpyHostToDevice);
+    // launches already queued may still read the previous lists
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(order);
+        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
+        set_error(std::string("set_window: ") + hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    (void)hipFree(plan->order);
+    plan->order = order;
+    for (int k = 0; k < 3; ++k) {
+        (void)hipFree(plan->tiles[k]);
+        plan->tiles[k] = tiles[k];
+        plan->ntiles[k] = (long long)htiles[k].size();
+    }
+    plan->window = window;
+    return EM_OK;
+}
+
+int em_ned2_plan_get_window(em_ned2_plan_t plan, int64_t* window) {
+    if (!plan || !window) {
+        set_error("null argument");
+        return EM_EINVAL;
+    }
+    *window = plan->window;
     return EM_OK;
 }
 

@@ -88,21 +88,31 @@ class CSCMapping:
         return self._lazy("col_offsets")
 
     def to_csc_device(self, data: torch.Tensor) -> torch.Tensor:
-        return self._dev.to_csc(data)
+        """CSC values of device COO values. Blocks that are symmetric (scalar or diagonal
+        materials, and any elementwise combination of such blocks, e.g. E - K0**2 B) are
+        summed by the mirrored column gather, others by the staged one; both give the
+        plain gather's result bit for bit (up to the sign of zero sums)."""
+        data = data.contiguous()
+        if data.numel() == 0:
+            return self._dev.to_csc(data)
+        blocks = data.view(-1, 20, 20)
+        if bool(torch.equal(blocks, blocks.transpose(1, 2))):
+            return self._dev.to_csc(data, block_symmetric=True)
+        return self._dev.to_csc_staged(data, None, 0.0)
 
     def to_csc(self, data):
         from scipy.sparse import csc_matrix
         if isinstance(data, torch.Tensor):
-            vals = self._dev.to_csc(data.to(self._dev.device)).cpu().numpy()
+            vals = self.to_csc_device(data.to(self._dev.device)).cpu().numpy()
         else:
             arr = np.asarray(data)
             if np.iscomplexobj(arr):
                 t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex128))
-                vals = self._dev.to_csc(t.to(self._dev.device)).cpu().numpy()
+                vals = self.to_csc_device(t.to(self._dev.device)).cpu().numpy()
             else:
                 # real data: the imaginary part of the complex sum is zero
                 t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(torch.complex128)
-                vals = self._dev.to_csc(t.to(self._dev.device)).cpu().numpy().real.copy()
+                vals = self.to_csc_device(t.to(self._dev.device)).cpu().numpy().real.copy()
         return csc_matrix((vals, self.indices, self.indptr), shape=(self.N, self.N))
 
 

@@ -149,6 +149,16 @@ int em_ned2_plan_sizes(em_ned2_plan_t plan, int64_t* n_assy, int64_t* n_field,
  */
 int em_ned2_plan_set_tuning(em_ned2_plan_t plan, int32_t tile);
 int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* tile);
+/*
+ * The other launch-shape knob (no counterpart in the reference either): tets per window of
+ * the tile order (32 .. 16,777,216; default 1,024). The assembly kernels group each
+ * window's tets by vertex order into tiles and run a window's tiles on one XCD; smaller
+ * windows keep the blocks written together close in memory, too small ones leave tiles
+ * nearly empty. Rebuilds the tile lists on the host, uploads them and synchronises the
+ * device (call it between launches, not in a captured stream). Set by the autotuner.
+ */
+int em_ned2_plan_set_window(em_ned2_plan_t plan, int64_t window);
+int em_ned2_plan_get_window(em_ned2_plan_t plan, int64_t* window);
 
 /*
  * Element stiffness and mass blocks, reference-style (curlcurl.py:387-456):

@@ -47,11 +47,37 @@ def test_autotuner_picks_and_caches(delaunay_field, tmp_path, monkeypatch):
     eps, mu = _mats(plan.n_tets, np.random.default_rng(2))
     k0 = 2 * np.pi * 10e9 / C0
     r = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, warmup=1)
-    assert r["nb"] in (32, 64, 128) and not r["cached"] and set(r["ms"]) == {32, 64, 128}
-    assert plan.get_tuning() == r["nb"]
+    assert r["nb"] in (32, 64, 128) and r["window"] in autotune.WINDOWS and not r["cached"]
+    assert set(r["ms"]) == {f"{w}/{nb}" for w in autotune.WINDOWS for nb in (32, 64, 128)}
+    assert plan.get_tuning() == r["nb"] and plan.get_window() == r["window"]
     plan2 = Ned2Plan.from_field(delaunay_field)
+    assert plan2.get_window() == 1024
     r2 = autotune.tune(plan2, eps, mu, k0, "mixed")
     assert r2["cached"] and r2["nb"] == r["nb"] and plan2.get_tuning() == r["nb"]
+    assert plan2.get_window() == r["window"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_window_variants_identical(delaunay_field, precision):
+    """The window only reorders the tiles: K, M and A are the same bit for bit at every
+    window (including one larger than the mesh and one of 32 tets), and equal the oracle."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    plan = Ned2Plan.from_field(delaunay_field)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(5))
+    k0 = 2 * np.pi * 9e9 / C0
+    ref = None
+    for w in (1024, 32, 512, 4096, 1 << 20):
+        plan.set_window(w)
+        assert plan.get_window() == w
+        for tile in (32, 128):
+            plan.set_tuning(tile)
+            A = plan.assemble_a(eps, mu, k0, precision).cpu()
+            if ref is None:
+                ref = A
+            assert torch.equal(A, ref), (w, tile)
+    with pytest.raises(Exception):
+        plan.set_window(16)
 
 
 def test_uneven_multimesh_batch_one_launch():

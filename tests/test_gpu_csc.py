@@ -251,3 +251,34 @@ def test_long_columns_and_the_limit(n):
     if n == 1200:
         with pytest.raises(UnsupportedError, match="3277 tets \\(at most 3276"):
             CscMap(_field_plan(_fan_field(3277)))
+
+
+@pytest.mark.parametrize("case", ["scalar-diag", "full-full", "not-symmetric"])
+def test_dropin_to_csc_picks_a_gather_with_the_plain_result(wr90_field, case):
+    """CSCMapping.to_csc checks the blocks of the data it is handed. The assembly's blocks
+    are always symmetric -- the reference fills i <= j and mirrors (curlcurl.py:358-376),
+    full non-symmetric tensors included -- and so is E - K0^2 B: those go to the mirrored
+    gather. Data with blocks that are not symmetric (built by the caller) go to the staged
+    gather. Either way the values equal the plain per-column gather's bit for bit and the
+    oracle's to_csc of the same data within 1e-12."""
+    import torch
+    from emerge_amd.curlcurl import tet_mass_stiffness_matrices
+    rng = np.random.default_rng(23)
+    T = wr90_field.n_tets
+    kinds = ("full", "full") if case == "full-full" else ("scalar", "diag")
+    er, ur = ref_tensor(T, kinds[0], rng), ref_tensor(T, kinds[1], rng)
+    Evec, Bvec, cscmap = tet_mass_stiffness_matrices(wr90_field, er, ur, np.array([]))
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(wr90_field, er, ur, np.array([]))
+    K0 = 2 * np.pi * 9.5e9 / 299792458.0
+    data, ref_data = Evec - Bvec * K0 ** 2, dE - dB * K0 ** 2
+    if case == "not-symmetric":
+        upper = np.triu(np.ones((20, 20)), 1).astype(bool).ravel()
+        scale = np.where(np.tile(upper, T), 1.5, 1.0)
+        data, ref_data = data * scale, ref_data * scale
+    blocks = data.reshape(-1, 20, 20)
+    assert np.array_equal(blocks, blocks.transpose(0, 2, 1)) == (case != "not-symmetric")
+    A = cscmap.to_csc(data)
+    plain = cscmap._dev.to_csc(torch.from_numpy(data).cuda()).cpu().numpy()
+    assert np.array_equal(A.data, plain)
+    ref = oracle.CSCMap(rows, cols, wr90_field.n_field).to_csc(ref_data)
+    assert rel_fro(A.data, ref.data) <= 1e-12

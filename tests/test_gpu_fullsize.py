@@ -1,0 +1,279 @@
+"""The configurations at their stated sizes, on one GPU.
+
+Oracle comparisons run on sampled slices (the oracle's COO arrays of a whole multi-million
+tet mesh do not fit a test's budget); the whole matrices are checked through properties
+that hold at any size: mixed against FP64 over every entry, exact symmetry of every element
+block, the CSC pattern's order and symmetry, and the sum of all values (a checksum of
+checksums) between the COO and CSC forms.
+
+* Delaunay bench mesh (400k seeded points, 2,815,239 tets), lossy scalar eps_r, diagonal
+  mu_r, 10 GHz;
+* frequency sweep: 551k-tet mesh, 64 frequencies over 8-12 GHz in one CSC graph replay;
+* 32 meshes of 5k-400k tets (log-uniform), one launch, per-tet complex eps_r;
+* backward on a 1M-tet mesh: dL/d eps_r, dL/d mu_r^-1 for L = Re(lam^H A x).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+C0 = 299792458.0
+K0 = 2 * np.pi * 10e9 / C0
+
+
+def _rel_dev(a, b, chunk=1 << 26) -> float:
+    """Relative Frobenius distance of two device arrays, accumulated in FP64 by chunks."""
+    import torch
+    num = den = 0.0
+    for s in range(0, b.numel(), chunk):
+        x = b[s:s + chunk].to(torch.complex128)
+        d = a[s:s + chunk].to(torch.complex128) - x
+        num += float(torch.sum(d.real ** 2 + d.imag ** 2))
+        den += float(torch.sum(x.real ** 2 + x.imag ** 2))
+    return (num / den) ** 0.5
+
+
+def _diag_ref(values):
+    """(3, 3, n) reference-layout tensors from per-tet diagonal values (n,) or (n, 3)."""
+    v = np.asarray(values)
+    n = v.shape[0]
+    out = np.zeros((3, 3, n), complex)
+    for d in range(3):
+        out[d, d] = v if v.ndim == 1 else v[:, d]
+    return out
+
+
+@pytest.fixture(scope="module")
+def bench_mesh():
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+    f = Nedelec2Field(delaunay_box_mesh(400_000, seed=2024))
+    T = f.n_tets
+    rng = np.random.default_rng(2025)
+    eps = rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T))
+    mu = rng.uniform(1, 2, (T, 3)).astype(complex)
+    return f, eps, mu
+
+
+def test_bench_mesh_full_matrix(bench_mesh):
+    """Every entry: mixed within 1e-6 of FP64; every 20x20 block of K and M exactly
+    symmetric (the reference mirrors i <= j, curlcurl.py:358-376); the CSC pattern sorted,
+    duplicate-free and structurally symmetric; the CSC values sum to the COO values' sum."""
+    import torch
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    f, eps, mu = bench_mesh
+    T = f.n_tets
+    assert T == 2_815_239
+    plan = Ned2Plan.from_field(f)
+    e, m = plan.material(torch.from_numpy(eps)), plan.material(torch.from_numpy(mu))
+    a64 = plan.assemble_a(e, m, K0, "fp64")
+    a32 = plan.assemble_a(e, m, K0, "mixed")
+    err = _rel_dev(a32, a64)
+    print(f"mixed vs fp64 over {a64.numel()} entries: {err:.3e}", flush=True)
+    assert err <= 1e-6
+    del a32
+    cm = CscMap(plan)
+    csc = cm.to_csc(a64, block_symmetric=True)
+    s_coo, s_csc = a64.sum(), csc.sum()
+    assert abs(complex(s_csc - s_coo)) <= 1e-12 * float(a64.abs().sum())
+    del a64, csc
+    ex = cm.export("indptr", "indices")
+    indptr, indices = ex["indptr"], ex["indices"]
+    N = f.n_field
+    assert int(indptr[-1]) == cm.nnz and bool(torch.all(indptr[1:] >= indptr[:-1]))
+    col_len = indptr[1:] - indptr[:-1]
+    starts = torch.zeros(cm.nnz, dtype=torch.bool, device="cuda")
+    starts[indptr[:-1][col_len > 0]] = True
+    inc = indices[1:] > indices[:-1]
+    assert bool(torch.all(inc | starts[1:]))  # strictly ascending rows inside each column
+    assert torch.equal(torch.bincount(indices, minlength=N), col_len)  # symmetric pattern
+    del indptr, indices, starts, inc
+    K, M = plan.assemble_km(e, m, "fp64")
+    for X in (K, M):
+        B = X.view(T, 20, 20)
+        for s in range(0, T, 200_000):
+            blk = B[s:s + 200_000]
+            assert torch.equal(blk, blk.transpose(1, 2)), s
+
+
+def test_bench_mesh_strided_sample_vs_oracle(bench_mesh):
+    """Every 53rd tet of the bench mesh (53,118 tets spread over the whole mesh, assembled
+    through the tetids compaction): FP64 within 1e-12 of the local-frame oracle, the
+    reference-arithmetic mode within 1e-14 of the verbatim oracle, mixed within 1e-6."""
+    from emerge_amd.assembly import Ned2Plan
+    f, eps, mu = bench_mesh
+    T = f.n_tets
+    ids = np.arange(0, T, 53, dtype=np.int64)
+    er, ur = _diag_ref(eps), _diag_ref(mu)
+    m = f.mesh
+    args = (m.nodes, m.tets, m.tris, m.edges, f.tet_to_field, m.tet_to_edge, m.tet_to_tri,
+            ur, er, ids, f.dofcodes3d)
+    El, Bl, rows, cols = oracle.matrix_builder(*args, local_frame=True)
+    Ev, Bv, _, _ = oracle.matrix_builder(*args, local_frame=False)
+    plan = Ned2Plan.from_field(f, tetids=ids)
+    r, c = plan.rowcol()
+    assert np.array_equal(r.cpu().numpy(), rows) and np.array_equal(c.cpu().numpy(), cols)
+    Al, Av = El - K0 ** 2 * Bl, Ev - K0 ** 2 * Bv
+    A = plan.assemble_a(er, ur, K0, "fp64").cpu().numpy()
+    Ar = plan.assemble_a(er, ur, K0, "fp64_ref").cpu().numpy()
+    A32 = plan.assemble_a(er, ur, K0, "mixed").cpu().numpy().astype(complex)
+    d = {"fp64_vs_local": rel_fro(A, Al), "ref_vs_verbatim": rel_fro(Ar, Av),
+         "mixed_vs_local": rel_fro(A32, Al), "fp64_vs_verbatim": rel_fro(A, Av)}
+    print({k: f"{v:.3e}" for k, v in d.items()}, flush=True)
+    assert d["fp64_vs_local"] <= 1e-12
+    assert d["ref_vs_verbatim"] <= 1e-14
+    assert d["mixed_vs_local"] <= 1e-6
+
+
+def test_sweep_551k_x64_csc_columns_vs_oracle():
+    """The CSC sweep graph on the 551k-tet sweep mesh (eps_r = mu_r = 1, 64 frequencies,
+    complex64): 2,000 sampled columns of A_f at f = 0, 31, 63 against the oracle's COO
+    values of the same columns summed by row in NumPy (within 1e-6 of FP64)."""
+    import torch
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    from emerge_amd.graph import CscSweepGraph
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=2027))
+    T, N = f.n_tets, f.n_field
+    plan = Ned2Plan.from_field(f)
+    cm = CscMap(plan)
+    k0s = 2 * np.pi * np.linspace(8e9, 12e9, 64) / C0
+    one = plan.material(1.0)
+    sg = CscSweepGraph(plan, cm, one, one, k0s, "mixed")
+    out = sg.replay()
+    torch.cuda.synchronize()
+    ex = cm.export("indptr", "indices")
+    indptr, indices = ex["indptr"].cpu().numpy(), ex["indices"].cpu().numpy()
+    I = _diag_ref(np.ones(T))
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, I, I, [], local_frame=True)
+    sample = np.sort(np.random.default_rng(7).choice(N, 2000, replace=False))
+    sel = np.isin(cols, sample)
+    r, c, e, b = rows[sel], cols[sel], dE[sel], dB[sel]
+    del dE, dB, rows, cols, sel
+    order = np.lexsort((r, c))
+    r, c, e, b = r[order], c[order], e[order], b[order]
+    first = np.r_[True, (r[1:] != r[:-1]) | (c[1:] != c[:-1])]
+    at = np.flatnonzero(first)
+    er_sum, eb_sum = np.add.reduceat(e, at), np.add.reduceat(b, at)
+    pos = np.concatenate([np.arange(indptr[j], indptr[j + 1]) for j in sample])
+    assert np.array_equal(indices[pos], r[at]) and np.array_equal(np.repeat(
+        sample, indptr[sample + 1] - indptr[sample]), c[at])
+    for fi in (0, 31, 63):
+        want = er_sum - k0s[fi] ** 2 * eb_sum
+        got = out[fi][torch.as_tensor(pos, device="cuda")].cpu().numpy().astype(complex)
+        err = rel_fro(got, want)
+        print(f"f{fi}: sampled-column error {err:.3e}", flush=True)
+        assert err <= 1e-6
+
+
+def test_batch_32_meshes_vs_oracle():
+    """32 meshes of 5k-400k tets (log-uniform sizes, seeded), one plan, one launch: every
+    mesh's slice on every 9th tet equals the oracle's assembly of that mesh (FP64 1e-12,
+    mixed 1e-6), and mixed is within 1e-6 of FP64 over each mesh's whole slice."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    rng = np.random.default_rng(2028)
+    sizes = np.exp(rng.uniform(np.log(5e3), np.log(4e5), 32)).astype(int)
+    fields = [Nedelec2Field(delaunay_mesh_with_tets(int(s), seed=2124 + i))
+              for i, s in enumerate(sizes)]
+    plan = Ned2Plan.from_fields(fields)
+    T = plan.n_tets
+    eps = torch.tensor(rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T)),
+                       dtype=torch.complex128)
+    a64 = plan.assemble_a(eps, 1.0, K0, "fp64")
+    a32 = plan.assemble_a(eps, 1.0, K0, "mixed")
+    eps_np = eps.numpy()
+    worst = 0.0
+    for i, f in enumerate(fields):
+        t0, t1 = int(plan.tet_offsets[i]), int(plan.tet_offsets[i + 1])
+        worst = max(worst, _rel_dev(a32[400 * t0:400 * t1], a64[400 * t0:400 * t1]))
+        ids = np.arange(0, f.n_tets, 9, dtype=np.int64)
+        m = f.mesh
+        er, ur = _diag_ref(eps_np[t0:t1]), _diag_ref(np.ones(f.n_tets))
+        dE, dB, _, _ = oracle.matrix_builder(m.nodes, m.tets, m.tris, m.edges, f.tet_to_field,
+                                             m.tet_to_edge, m.tet_to_tri, ur, er, ids,
+                                             f.dofcodes3d, local_frame=True)
+        idx = torch.as_tensor(((t0 + ids)[:, None] * 400 + np.arange(400)).ravel(),
+                              device="cuda")
+        want = dE - K0 ** 2 * dB
+        assert rel_fro(a64[idx].cpu().numpy(), want) <= 1e-12, i
+        assert rel_fro(a32[idx].cpu().numpy().astype(complex), want) <= 1e-6, i
+    print(f"32 meshes, {T} tets; worst per-mesh mixed vs fp64 {worst:.3e}", flush=True)
+    assert worst <= 1e-6
+
+
+def test_backward_1M_tets():
+    """Backward on a 1M-tet mesh: the gradients of 256 sampled tets against the FP64
+    reference gradient from the oracle's unit-material blocks of those tets, and against
+    central differences (h = 1e-6, real and imaginary parts) of the per-tet terms of L
+    through the GPU forward on a plan of the sampled tets."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.autograd import assemble_a, bilinear_form
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(1_000_000, seed=2029))
+    T, N = f.n_tets, f.n_field
+    plan = Ned2Plan.from_field(f)
+    rng = np.random.default_rng(2029)
+    eps = torch.tensor(rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T)),
+                       dtype=torch.complex128, device="cuda")
+    muinv = torch.tensor(1.0 / rng.uniform(1, 2, (T, 3)), dtype=torch.complex128, device="cuda")
+    lam = torch.tensor(rng.normal(size=N) + 1j * rng.normal(size=N), device="cuda")
+    x = torch.tensor(rng.normal(size=N) + 1j * rng.normal(size=N), device="cuda")
+    rows, cols = plan.rowcol()
+    e = eps.clone().requires_grad_(True)
+    mi = muinv.clone().requires_grad_(True)
+    L = bilinear_form(assemble_a(plan, e, mi, K0, "fp64"), rows, cols, lam, x)
+    L.backward()
+    ge, gm = e.grad.cpu().numpy(), mi.grad.cpu().numpy()
+    del rows, cols
+    sample = np.sort(rng.choice(T, 256, replace=False)).astype(np.int64)
+    # FP64 reference gradient of the sampled tets (A is linear in each leaf)
+    m = f.mesh
+    I = _diag_ref(np.ones(T))
+
+    def unit_blocks(ur):
+        return oracle.matrix_builder(m.nodes, m.tets, m.tris, m.edges, f.tet_to_field,
+                                     m.tet_to_edge, m.tet_to_tri, ur, I, sample, f.dofcodes3d,
+                                     local_frame=True)
+
+    K0b, Mhat, r_s, c_s = unit_blocks(I)
+    lam_h, x_h = lam.cpu().numpy(), x.cpu().numpy()
+    G = lam_h[r_s] * np.conj(x_h[c_s])
+    ref_e = -K0 ** 2 * (Mhat * G).reshape(-1, 400).sum(axis=1)
+    assert np.linalg.norm(ge[sample] - ref_e) <= 1e-12 * np.linalg.norm(ref_e)
+    for d in range(3):
+        mu = np.ones((T, 3))
+        mu[:, d] = 0.5  # mu^-1 = I + e_d e_d^T
+        Kd = unit_blocks(_diag_ref(mu))[0]
+        ref_m = ((Kd - K0b) * G).reshape(-1, 400).sum(axis=1)
+        assert np.linalg.norm(gm[sample, d] - ref_m) <= 1e-10 * np.linalg.norm(ref_m), d
+    # central differences of the sampled tets' own terms of L (the other terms do not
+    # depend on these leaves), through the forward kernel on a plan of those tets
+    sub = Ned2Plan.from_field(f, tetids=sample)
+    rs, cs = sub.rowcol()
+    h = 1e-6
+
+    def terms(ev, mv):
+        A = sub.assemble_a(sub.material(ev), sub.material(mv, inverse=True), K0, "fp64")
+        w = torch.conj(lam[rs]) * A * x[cs]
+        return torch.real(w).reshape(-1, 400).sum(dim=1)
+
+    fd_e = np.zeros(256, complex)
+    fd_m = np.zeros(256, complex)
+    for unit in (1.0, 1j):
+        dE = torch.zeros(T, dtype=torch.complex128, device="cuda")
+        dE[torch.as_tensor(sample, device="cuda")] = unit * h
+        d = ((terms(eps + dE, muinv) - terms(eps - dE, muinv)) / (2 * h)).cpu().numpy()
+        fd_e += d if unit == 1.0 else 1j * d
+        dM = torch.zeros((T, 3), dtype=torch.complex128, device="cuda")
+        dM[torch.as_tensor(sample, device="cuda"), 2] = unit * h
+        d = ((terms(eps, muinv + dM) - terms(eps, muinv - dM)) / (2 * h)).cpu().numpy()
+        fd_m += d if unit == 1.0 else 1j * d
+    ee = np.linalg.norm(fd_e - ge[sample]) / np.linalg.norm(ge[sample])
+    em = np.linalg.norm(fd_m - gm[sample, 2]) / np.linalg.norm(gm[sample, 2])
+    print(f"1M tets: central differences vs backward: eps {ee:.2e}, mu^-1 {em:.2e}", flush=True)
+    assert ee <= 1e-6 and em <= 1e-6

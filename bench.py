@@ -645,11 +645,13 @@ def target_run(args) -> dict:
     log(f"target: {T} tets")
     res = {"n_tets": T, "points": args.target_points}
     a32 = torch.empty(plan.nnz_coo, dtype=torch.complex64, device="cuda")
-    autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
+    t32 = autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
     ms32, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "mixed", out=a32), 10, 2)
     a64 = torch.empty(plan.nnz_coo, dtype=torch.complex128, device="cuda")
-    autotune.tune(plan, er, ur, k0, "fp64", out=a64[None], reps=3)
+    t64 = autotune.tune(plan, er, ur, k0, "fp64", out=a64[None], reps=3)
     ms64, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "fp64", out=a64), 10, 2)
+    res["tuning"] = {"mixed": {"tile": t32["nb"], "window": t32["window"]},
+                     "fp64": {"tile": t64["nb"], "window": t64["window"]}}
     autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
     plan.assemble_a(er, ur, k0, "mixed", out=a32)
     torch.cuda.synchronize()

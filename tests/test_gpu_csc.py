@@ -1,6 +1,8 @@
 """GPU CSC map (precompute_csc_pattern / to_csc, csc_cast.py:65-177) against the oracle's
 restatement and against SciPy's COO->CSC conversion; and the drop-in entry point
 (curlcurl.py:222-270 + assembler.py:494) end to end."""
+import time
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -215,7 +217,7 @@ def test_staged_gather_non_symmetric_blocks(delaunay_field, mesh, precision):
     assert torch.equal(sub.to_csc_staged(K, M, k0, work=work), sub.to_csc_combine(K, M, k0))
 
 
-@pytest.mark.parametrize("n", [102, 103, 614, 615, 1200])
+@pytest.mark.parametrize("n", [102, 103, 614, 615, 1200, 3276])
 def test_long_columns_and_the_limit(n):
     """A DOF in n tets gives a column of 20 n COO entries. Up to 102 tets (2,040 entries)
     the column kernel sorts them in LDS; from 103 on the long-column kernel ranks them,
@@ -232,7 +234,12 @@ def test_long_columns_and_the_limit(n):
     T = field.n_tets
     er, ur = ref_tensor(T, "scalar", rng), ref_tensor(T, "diag", rng)
     plan = _field_plan(field)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     cm = CscMap(plan)
+    torch.cuda.synchronize()
+    print(f"map of a {n}-tet fan ({20 * n} entries in the axis columns): "
+          f"{(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
     dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(field, er, ur, np.array([]),
                                                             local_frame=True)
     ref = oracle.CSCMap(rows, cols, field.n_field)

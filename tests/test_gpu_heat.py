@@ -1,6 +1,8 @@
 """GPU parity of the heat-conduction (Lagrange-2) assembly (heat.hip, emerge_amd.heat)
 against the oracle restatement of grad.py, the CSC pattern against SciPy, and a global
 energy pin of the assembled conductance matrix."""
+import time
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -128,7 +130,7 @@ def test_heat_csc_and_energy(wr90_field):
     assert rel_fro(Md, M.cpu().numpy()) <= 1e-14
 
 
-@pytest.mark.parametrize("n", [205, 1228, 1229, 2000])
+@pytest.mark.parametrize("n", [205, 1228, 1229, 2000, 6553])
 def test_heat_long_columns_and_the_limit(n):
     """The Lagrange-2 DOFs of a fan's axis (its two vertices and its midpoint) lie in all
     n tets: columns of 10 n COO entries, past the column kernel's 2,048-entry LDS sort
@@ -147,7 +149,12 @@ def test_heat_long_columns_and_the_limit(n):
     rng = np.random.default_rng(n)
     K, M = plan.assemble(cond=torch.tensor(rng.uniform(1, 3, T)),
                          rho_cp=torch.tensor(rng.uniform(1, 2, T)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     cm = HeatCscMap(plan)
+    torch.cuda.synchronize()
+    print(f"heat map of a {n}-tet fan ({10 * n} entries in the axis columns): "
+          f"{(time.perf_counter() - t0) * 1e3:.1f} ms", flush=True)
     r, cc = plan.rowcol()
     S = sp.coo_matrix((K.cpu().numpy(), (r.cpu().numpy(), cc.cpu().numpy())),
                       shape=(f.n_field, f.n_field)).tocsc()

@@ -72,6 +72,15 @@ def rel_err_chunked(a32, a64, chunk=1 << 26) -> float:
     return (num / den) ** 0.5
 
 
+def tuning_record(r: dict) -> dict:
+    """The autotuner's choice and the fastest time it measured per window."""
+    best = {}
+    for k, ms in r["ms"].items():
+        w = k.split("/")[0]
+        best[w] = min(ms, best.get(w, ms))
+    return {"tile": r["nb"], "window": r["window"], "best_ms_by_window": best}
+
+
 def host_cpu_info() -> dict:
     """The host's CPU counts: physical cores (distinct core/socket pairs of ``lscpu -p``),
     logical CPUs (``nproc``), the CPUs this process may run on, and the thread share a
@@ -240,6 +249,38 @@ def suite_sweep(args) -> dict:
     return res
 
 
+def suite_ref(args) -> dict:
+    """The reference-arithmetic FP64 mode (EM_PREC_FP64_REF: 11-point rule in absolute
+    coordinates, curlcurl.py:285-380) against the exact-integral FP64 kernel on a ~500k-tet
+    Delaunay mesh: K and M (the reference's dataE, dataB), complex128, per-tet complex
+    eps_r, diagonal mu_r; and their relative Frobenius distance on A at 10 GHz."""
+    import numpy as np
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=args.seed + 3))
+    plan = Ned2Plan.from_field(f)
+    T = plan.n_tets
+    eps, mu = _mats(T, np.random.default_rng(args.seed + 21))
+    eps, mu = plan.material(eps), plan.material(mu)
+    ms = {}
+    for prec in ("fp64_ref", "fp64"):
+        ms[prec], _ = time_launches(lambda: plan.assemble_km(eps, mu, prec), 3, 1)
+        log(f"ref suite: {prec} K and M {ms[prec]:.3f} ms")
+    k0 = 2 * np.pi * 10e9 / C0
+    a_ref = plan.assemble_a(eps, mu, k0, "fp64_ref")
+    a_ex = plan.assemble_a(eps, mu, k0, "fp64")
+    gap = float(torch.linalg.norm(a_ref - a_ex) / torch.linalg.norm(a_ex))
+    del a_ref, a_ex
+    plan.close()
+    torch.cuda.empty_cache()
+    return {"n_tets": T, "output": "K and M, complex128 COO",
+            "ms_fp64_ref": ms["fp64_ref"], "ms_fp64": ms["fp64"],
+            "tets_per_s_fp64_ref": T / (ms["fp64_ref"] * 1e-3),
+            "tets_per_s_fp64": T / (ms["fp64"] * 1e-3),
+            "rel_frobenius_ref_vs_exact_A": gap}
+
+
 def suite_batch(args) -> dict:
     """32 Delaunay meshes, tet counts log-uniform in [5k, 400k], per-tet complex eps_r,
     mu_r = 1, mixed precision, one launch; per-mesh error against the FP64 kernel (which
@@ -275,6 +316,7 @@ def suite_batch(args) -> dict:
     return {"n_meshes": 32, "n_tets": T, "tets_per_mesh": [f.n_tets for f in fields],
             "ms_per_launch": ms, "tets_per_s": T / (ms * 1e-3),
             "gb_per_s": T * b / (ms * 1e-3) / 1e9, "tuned_nb": tune["nb"], "tuned_window": tune["window"],
+
             "rel_err_vs_fp64_per_mesh": errs, "max_rel_err": max(errs)}
 
 
@@ -650,8 +692,7 @@ def target_run(args) -> dict:
     a64 = torch.empty(plan.nnz_coo, dtype=torch.complex128, device="cuda")
     t64 = autotune.tune(plan, er, ur, k0, "fp64", out=a64[None], reps=3)
     ms64, _ = time_launches(lambda: plan.assemble_a(er, ur, k0, "fp64", out=a64), 10, 2)
-    res["tuning"] = {"mixed": {"tile": t32["nb"], "window": t32["window"]},
-                     "fp64": {"tile": t64["nb"], "window": t64["window"]}}
+    res["tuning"] = {"mixed": tuning_record(t32), "fp64": tuning_record(t64)}
     autotune.tune(plan, er, ur, k0, "mixed", out=a32[None], reps=3)
     plan.assemble_a(er, ur, k0, "mixed", out=a32)
     torch.cuda.synchronize()
@@ -697,7 +738,7 @@ def main() -> int:
                          "32-mesh batch and the backward on 1M tets")
     ap.add_argument("--only", default="",
                     help="with --suite: comma list among sweep,batch,backward,csc_pec,"
-                         "surface_interp,heat,materials")
+                         "surface_interp,heat,materials,ref")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -739,7 +780,7 @@ def main() -> int:
     from emerge_amd import autotune
     a32 = torch.empty(nnz, dtype=torch.complex64, device="cuda")
     r = autotune.tune(plan, eps, mu, k0, "mixed", out=a32[None])
-    tuning = {"mixed": {"tile": r["nb"], "window": r["window"]}}
+    tuning = {"mixed": tuning_record(r)}
     ms32, wall32 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32),
                                  args.steps, args.warmup)
     log(f"mixed: {ms32:.3f} ms/launch")
@@ -748,7 +789,7 @@ def main() -> int:
     if not args.no_fp64:
         a64 = torch.empty(nnz, dtype=torch.complex128, device="cuda")
         r = autotune.tune(plan, eps, mu, k0, "fp64", out=a64[None])
-        tuning["fp64"] = {"tile": r["nb"], "window": r["window"]}
+        tuning["fp64"] = tuning_record(r)
         ms64, wall64 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "fp64", out=a64),
                                      args.steps, args.warmup)
         log(f"fp64: {ms64:.3f} ms/launch")
@@ -851,7 +892,8 @@ def main() -> int:
                     ("csc_pec", "csc_pec_500k", suite_csc_pec),
                     ("surface_interp", "surface_interp_500k", suite_surface_interp),
                     ("heat", "heat_2.8M", suite_heat),
-                    ("materials", "materials_2.8M", suite_materials))
+                    ("materials", "materials_2.8M", suite_materials),
+                    ("ref", "fp64_ref_500k", suite_ref))
             only = set(args.only.split(",")) if args.only else None
             for short, key, fn in runs:
                 if only is None or short in only:

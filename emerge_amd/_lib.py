@@ -144,6 +144,10 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m emerge_amd.build`")
+    # torch ships its own libamdhip64; loading it first makes the library bind to that
+    # runtime (same soname), so torch's device pointers and streams are valid in it. The
+    # other order leaves two HIP runtimes in the process, and the second sees no device.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(L, name)

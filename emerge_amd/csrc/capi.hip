@@ -155,6 +155,13 @@ namespace {
 // 1,024-tet windows launch 4% faster than 16,384-tet ones and 512-tet windows 6%, on an
 // 8.3M-tet mesh 512-tet windows are 30% slower (tiles of ~20 tets). The autotuner picks
 // the window per mesh (em_ned2_plan_set_window); plan creation uses 1,024.
+//
+// Giving tets to XCDs by where their output lands instead (tet -> the XCD x whose 4 KiB
+// output chunks, chunk index % 8 == x, hold most of its block) was tried: stores alone
+// run 6% faster when each XCD writes chunks of one residue, but the kernel gained nothing
+// (2.8M tets, complex64: within +-1%, also with uniform materials), because the 8 tets
+// sharing a 128-byte line of the per-tet inputs then run on ~6 XCDs: 1,125 MB fetched per
+// launch instead of 300 MB.
 void build_tiles(const std::vector<unsigned char>& code, long long window, std::vector<int>& horder,
                  std::vector<std::vector<int2>>& htiles) {
     constexpr int kXcds = 8;
@@ -188,6 +195,48 @@ void build_tiles(const std::vector<unsigned char>& code, long long window, std::
             for (size_t j = 0; j < per_xcd[k][x].size(); ++j)
                 htiles[k][j * kXcds + x] = per_xcd[k][x][j];
     }
+}
+
+// Rebuild a plan's tile lists for another window: upload, wait for launches still
+// reading the old lists, swap.
+int rebuild_tiles(em_ned2_plan_s* plan, long long window) {
+    std::vector<int> horder;
+    std::vector<std::vector<int2>> htiles;
+    build_tiles(plan->code, window, horder, htiles);
+    DeviceGuard g(plan->device);
+    int* order = nullptr;
+    int2* tiles[3] = {nullptr, nullptr, nullptr};
+    int st;
+    if ((st = dmalloc(&order, horder.size())) || (st = dmalloc(&tiles[0], htiles[0].size())) ||
+        (st = dmalloc(&tiles[1], htiles[1].size())) || (st = dmalloc(&tiles[2], htiles[2].size()))) {
+        (void)hipFree(order);
+        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
+        return st;
+    }
+    hipError_t e = horder.empty() ? hipSuccess
+                                  : hipMemcpy(order, horder.data(), horder.size() * sizeof(int),
+                                              hipMemcpyHostToDevice);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+        if (!htiles[k].empty())
+            e = hipMemcpy(tiles[k], htiles[k].data(), htiles[k].size() * sizeof(int2),
+                          hipMemcpyHostToDevice);
+    // launches already queued may still read the previous lists
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(order);
+        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
+        set_error(std::string("tile lists: ") + hipGetErrorString(e));
+        return EM_EHIP;
+    }
+    (void)hipFree(plan->order);
+    plan->order = order;
+    for (int k = 0; k < 3; ++k) {
+        (void)hipFree(plan->tiles[k]);
+        plan->tiles[k] = tiles[k];
+        plan->ntiles[k] = (long long)htiles[k].size();
+    }
+    plan->window = window;
+    return EM_OK;
 }
 
 }  // namespace
@@ -591,43 +640,7 @@ int em_ned2_plan_set_window(em_ned2_plan_t plan, int64_t window) {
         return EM_EINVAL;
     }
     if (window == plan->window) return EM_OK;
-    std::vector<int> horder;
-    std::vector<std::vector<int2>> htiles;
-    build_tiles(plan->code, window, horder, htiles);
-    DeviceGuard g(plan->device);
-    int* order = nullptr;
-    int2* tiles[3] = {nullptr, nullptr, nullptr};
-    int st;
-    if ((st = dmalloc(&order, horder.size())) || (st = dmalloc(&tiles[0], htiles[0].size())) ||
-        (st = dmalloc(&tiles[1], htiles[1].size())) || (st = dmalloc(&tiles[2], htiles[2].size()))) {
-        (void)hipFree(order);
-        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
-        return st;
-    }
-    hipError_t e = horder.empty() ? hipSuccess
-                                  : hipMemcpy(order, horder.data(), horder.size() * sizeof(int),
-                                              hipMemcpyHostToDevice);
-    for (int k = 0; k < 3 && e == hipSuccess; ++k)
-        if (!htiles[k].empty())
-            e = hipMemcpy(tiles[k], htiles[k].data(), htiles[k].size() * sizeof(int2),
-                          hipMemcpyHostToDevice);
-    // launches already queued may still read the previous lists
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        (void)hipFree(order);
-        for (int k = 0; k < 3; ++k) (void)hipFree(tiles[k]);
-        set_error(std::string("set_window: ") + hipGetErrorString(e));
-        return EM_EHIP;
-    }
-    (void)hipFree(plan->order);
-    plan->order = order;
-    for (int k = 0; k < 3; ++k) {
-        (void)hipFree(plan->tiles[k]);
-        plan->tiles[k] = tiles[k];
-        plan->ntiles[k] = (long long)htiles[k].size();
-    }
-    plan->window = window;
-    return EM_OK;
+    return rebuild_tiles(plan, window);
 }
 
 int em_ned2_plan_get_window(em_ned2_plan_t plan, int64_t* window) {

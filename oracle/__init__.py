@@ -19,13 +19,16 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libned2_oracle.so")
+_DEFAULT_LIB = os.path.join(_HERE, "libned2_oracle.so")
+# EMERGE_ORACLE_LIB selects another build of the same sources (the sanitizer build of
+# `make -C oracle sanitize`); it is never built implicitly.
+_LIB_PATH = os.environ.get("EMERGE_ORACLE_LIB", _DEFAULT_LIB)
 _lib = None
 
 
 def build() -> str:
     subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
-    return _LIB_PATH
+    return _DEFAULT_LIB
 
 
 def _load():
@@ -33,6 +36,8 @@ def _load():
     if _lib is not None:
         return _lib
     if not os.path.exists(_LIB_PATH):
+        if _LIB_PATH != _DEFAULT_LIB:
+            raise OSError(f"EMERGE_ORACLE_LIB={_LIB_PATH} does not exist")
         build()
     lib = ctypes.CDLL(_LIB_PATH)
     P = ctypes.c_void_p

@@ -1,7 +1,7 @@
 """Device-resident volume part of ``Assembler.assemble_freq_matrix``.
 
 Follows src/emerge/_emerge/physics/microwave/assembly/assembler.py:423-494 (the same
-steps in ``assemble_scattering_matrix``, :819-868; ``assemble_eig_matrix``, :1115-1153,
+steps in ``assemble_scattering_matrix``, :808-868; ``assemble_eig_matrix``, :1115-1153,
 converts K and M separately instead of combining them):
 
 1. ``W0 = 2*pi*f``, ``K0 = W0 / C0`` (:423-424, const.py:20);
@@ -118,6 +118,18 @@ class VolumeAssembler:
         else:
             data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
         return FreqMatrix(data, cm, conductor_tets, fd, K0)
+
+    def assemble_scattering_matrix(self, er, ur, tand, cond, frequency: float,
+                                   cache_matrices: bool = True,
+                                   frequency_dependent: bool = False) -> FreqMatrix:
+        """The volume step of ``assemble_scattering_matrix`` (assembler.py:776-868). Its
+        lines 808-868 repeat :423-494 step for step (material preparation, conductor tets,
+        the ``cached_matrices`` test, ``to_csc(E - B K0**2)``) on the same caches, so the
+        two entry points reuse each other's element matrices as the reference's do. The
+        scattered-field excitation that follows it (:878 on) is the scattered form of
+        ``emerge_amd.robin``'s port force."""
+        return self.assemble_freq_matrix(er, ur, tand, cond, frequency, cache_matrices,
+                                         frequency_dependent)
 
     def assemble_eig_matrices(self, er, ur, tand, cond, frequency: float):
         """The volume step of ``assemble_eig_matrix`` (assembler.py:1115-1153): the same

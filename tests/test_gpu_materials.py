@@ -146,6 +146,43 @@ def test_volume_assembler_flow_and_caching(wr90_field):
     assert rel_fro(a32.cpu().numpy().astype(complex), a64.cpu().numpy()) < 1e-6
 
 
+def test_volume_assembler_scattering_shares_the_cache(wr90_field):
+    """assemble_scattering_matrix's volume step (assembler.py:808-868) is the frequency
+    path's on the same caches: equal results bit for bit, element matrices built once and
+    reused by either entry point while nothing is frequency dependent, and the oracle's A."""
+    import torch
+    import oracle
+    from emerge_amd.assembler import VolumeAssembler
+    f = wr90_field
+    T = f.n_tets
+    rng = np.random.default_rng(9)
+    er = np.zeros((3, 3, T), complex)
+    ur = np.zeros((3, 3, T), complex)
+    tand = np.zeros((3, 3, T), complex)
+    cond = np.zeros((3, 3, T), complex)
+    for d in range(3):
+        er[d, d], tand[d, d], ur[d, d] = rng.uniform(1, 4, T), rng.uniform(0, 0.01, T), 1.0
+        cond[d, d] = np.where(np.arange(T) % 89 == 0, 1e9, 0.0)  # PEC-only conductors
+    va = VolumeAssembler(f)
+    s1 = va.assemble_scattering_matrix(er, ur, tand, cond, 9.1e9)
+    f1 = va.assemble_freq_matrix(er, ur, tand, cond, 9.1e9)
+    s2 = va.assemble_scattering_matrix(er, ur, tand, cond, 10.3e9)
+    assert va.assemblies == 1
+    assert torch.equal(s1.data, f1.data)
+    assert np.array_equal(s1.conductor_tets, f1.conductor_tets)
+    fresh = VolumeAssembler(f).assemble_freq_matrix(er, ur, tand, cond, 10.3e9)
+    assert torch.equal(s2.data, fresh.data)
+    ref, ids, _ = om.prepare_materials(er, tand, cond, 10.3e9)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, ref, ur, ids)
+    k0 = 2 * np.pi * 10.3e9 / 299792458
+    A_ref = oracle.CSCMap(rows, cols, f.n_field).to_csc(dE - dB * k0 ** 2)
+    A = s2.to_scipy()
+    assert np.array_equal(A.indptr, A_ref.indptr) and np.array_equal(A.indices, A_ref.indices)
+    assert rel_fro(A.data, A_ref.data) <= 1e-12
+    va.assemble_scattering_matrix(er, ur, tand, cond, 10.3e9, cache_matrices=False)
+    assert va.assemblies == 2
+
+
 def test_volume_assembler_eig_matrices(wr90_field):
     """assemble_eig_matrix's volume step (assembler.py:1115-1153): K and M converted to CSC
     separately, conductor tets removed; equal to the oracle's."""

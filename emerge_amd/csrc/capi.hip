@@ -159,9 +159,10 @@ namespace {
 // Giving tets to XCDs by where their output lands instead (tet -> the XCD x whose 4 KiB
 // output chunks, chunk index % 8 == x, hold most of its block) was tried: stores alone
 // run 6% faster when each XCD writes chunks of one residue, but the kernel gained nothing
-// (2.8M tets, complex64: within +-1%, also with uniform materials), because the 8 tets
-// sharing a 128-byte line of the per-tet inputs then run on ~6 XCDs: 1,125 MB fetched per
-// launch instead of 300 MB.
+// (2.8M tets, complex64: within +-1%, also with uniform materials): a stores-only replica
+// of its tile walk is as fast with 80% of the bytes on the matching XCD as without, and
+// the 8 tets sharing a 128-byte line of the per-tet inputs then run on ~6 XCDs (1,125 MB
+// fetched per launch instead of 300 MB).
 void build_tiles(const std::vector<unsigned char>& code, long long window, std::vector<int>& horder,
                  std::vector<std::vector<int2>>& htiles) {
     constexpr int kXcds = 8;

@@ -281,6 +281,60 @@ def suite_ref(args) -> dict:
             "rel_frobenius_ref_vs_exact_A": gap}
 
 
+def suite_dropin(args) -> dict:
+    """The Python drop-in tet_mass_stiffness_matrices (curlcurl.py:222-270) end to end on a
+    ~500k-tet Delaunay mesh: host (3,3,T) complex128 er/ur in, host complex128 dataE/dataB
+    out (the reference's interface), so both PCIe directions are inside the timed call;
+    beside it the device-resident assembly of the same matrices and the bare copies."""
+    import numpy as np
+    import torch
+    from emerge_amd.curlcurl import tet_mass_stiffness_matrices
+    from emerge_amd.mesh import Nedelec2Field, delaunay_mesh_with_tets
+    f = Nedelec2Field(delaunay_mesh_with_tets(500_000, seed=args.seed + 3))
+    T = f.n_tets
+    rng = np.random.default_rng(args.seed + 23)
+    er = np.zeros((3, 3, T), np.complex128)
+    ur = np.zeros((3, 3, T), np.complex128)
+    eps = rng.uniform(1, 10, T) * (1 - 1j * rng.uniform(0, 0.02, T))
+    for d in range(3):
+        er[d, d] = eps
+        ur[d, d] = rng.uniform(1, 2, T)
+    cond = np.array([], np.int64)
+    _, _, cm = tet_mass_stiffness_matrices(f, er, ur, cond)  # plan, map and first copies
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dE, dB, _ = tet_mass_stiffness_matrices(f, er, ur, cond, cscmap=cm)
+    call_s = (time.perf_counter() - t0) / reps
+    from emerge_amd.curlcurl import plan_for
+    plan = plan_for(f, cond)
+    er_d, ur_d = plan.material(er, layout="ref"), plan.material(ur, layout="ref")
+    dev_ms, _ = time_launches(lambda: plan.assemble_km(er_d, ur_d, "fp64"), 5, 1)
+    K, M = plan.assemble_km(er_d, ur_d, "fp64")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        K.cpu(), M.cpu()
+    pageable_s = (time.perf_counter() - t0) / reps
+    from emerge_amd.assembly import to_host
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hK, hM = to_host(K), to_host(M)
+    d2h_s = (time.perf_counter() - t0) / reps
+    del hK, hM
+    out_bytes = 2 * K.numel() * 16
+    del K, M, dE, dB
+    torch.cuda.empty_cache()
+    return {"n_tets": T, "output": "dataE, dataB host complex128 (2 x 6,400 B/tet)",
+            "s_per_call": call_s, "tets_per_s_call": T / call_s,
+            "ms_device_assembly_km": dev_ms, "tets_per_s_device": T / (dev_ms * 1e-3),
+            "s_d2h_copies": d2h_s, "d2h_gb_per_s": out_bytes / d2h_s / 1e9,
+            "s_d2h_pageable": pageable_s, "d2h_pageable_gb_per_s": out_bytes / pageable_s / 1e9,
+            "note": "fresh host arrays per call, as the reference returns; the drop-in "
+                    "copies into pinned memory (s_d2h_copies), torch's .cpu() into "
+                    "pageable memory (s_d2h_pageable)"}
+
+
 def suite_batch(args) -> dict:
     """32 Delaunay meshes, tet counts log-uniform in [5k, 400k], per-tet complex eps_r,
     mu_r = 1, mixed precision, one launch; per-mesh error against the FP64 kernel (which
@@ -738,7 +792,7 @@ def main() -> int:
                          "32-mesh batch and the backward on 1M tets")
     ap.add_argument("--only", default="",
                     help="with --suite: comma list among sweep,batch,backward,csc_pec,"
-                         "surface_interp,heat,materials,ref")
+                         "surface_interp,heat,materials,ref,dropin")
     ap.add_argument("--target-points", type=int, default=0,
                     help="also run the large single-mesh case with this many points "
                          "(1200000 gives about 8M tets)")
@@ -893,7 +947,8 @@ def main() -> int:
                     ("surface_interp", "surface_interp_500k", suite_surface_interp),
                     ("heat", "heat_2.8M", suite_heat),
                     ("materials", "materials_2.8M", suite_materials),
-                    ("ref", "fp64_ref_500k", suite_ref))
+                    ("ref", "fp64_ref_500k", suite_ref),
+                    ("dropin", "dropin_500k", suite_dropin))
             only = set(args.only.split(",")) if args.only else None
             for short, key, fn in runs:
                 if only is None or short in only:

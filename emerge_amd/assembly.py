@@ -23,6 +23,18 @@ _PREC = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "fp64_ref": PREC_FP64_REF,
          PREC_FP64: PREC_FP64, PREC_MIXED: PREC_MIXED, PREC_FP64_REF: PREC_FP64_REF}
 
 
+def to_host(t: torch.Tensor) -> np.ndarray:
+    """A fresh NumPy copy of a device tensor, through pinned host memory: on one MI355X a
+    3.5 GB complex128 COO array takes ~0.88 s into pageable memory (4.0 GB/s, page faults
+    of the fresh array included) and ~0.09 s into pinned memory (39 GB/s). The array
+    keeps its pinned tensor alive; torch's host allocator reuses the block once both are
+    freed."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    torch.cuda.current_stream(t.device).synchronize()
+    return h.numpy()
+
+
 def _prec(p) -> int:
     """Precision name or code: ``"fp64"`` (exact element integrals, complex128),
     ``"mixed"`` (FP32 contraction, complex64) or ``"fp64_ref"`` (the reference's own

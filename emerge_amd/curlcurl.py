@@ -15,7 +15,9 @@ signature, argument meaning and return types of
 
 The work runs on the GPU: the HIP kernels compute the element blocks and the CSC map is
 built and applied on the device. Host arrays cross PCIe in both directions, since the
-reference's callers hold NumPy arrays. The device plan of a field is cached.
+reference's callers hold NumPy arrays; the returned arrays live in pinned host memory
+(a copy into pageable memory runs at a few GB/s and would make the call slower than the
+CPU reference). The device plan of a field is cached.
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ import weakref
 import numpy as np
 import torch
 
-from .assembly import CscMap, Ned2Plan
+from .assembly import CscMap, Ned2Plan, to_host
 
 _PLANS: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
 
@@ -62,8 +64,8 @@ class CSCMapping:
     def __init__(self, dev: CscMap):
         self._dev = dev
         arrs = dev.export("indptr", "indices")
-        self.indptr = arrs["indptr"].cpu().numpy()
-        self.indices = arrs["indices"].cpu().numpy()
+        self.indptr = to_host(arrs["indptr"])
+        self.indices = to_host(arrs["indices"])
         self.N = dev.N
         self.nnz = dev.nnz
         self.nnz_coo = dev.nnz_coo
@@ -73,7 +75,7 @@ class CSCMapping:
         return cls(CscMap(plan))
 
     def _lazy(self, name):
-        return self._dev.export(name)[name].cpu().numpy()
+        return to_host(self._dev.export(name)[name])
 
     @property
     def perm(self) -> np.ndarray:
@@ -103,16 +105,16 @@ class CSCMapping:
     def to_csc(self, data):
         from scipy.sparse import csc_matrix
         if isinstance(data, torch.Tensor):
-            vals = self.to_csc_device(data.to(self._dev.device)).cpu().numpy()
+            vals = to_host(self.to_csc_device(data.to(self._dev.device)))
         else:
             arr = np.asarray(data)
             if np.iscomplexobj(arr):
                 t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex128))
-                vals = self.to_csc_device(t.to(self._dev.device)).cpu().numpy()
+                vals = to_host(self.to_csc_device(t.to(self._dev.device)))
             else:
                 # real data: the imaginary part of the complex sum is zero
                 t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(torch.complex128)
-                vals = self.to_csc_device(t.to(self._dev.device)).cpu().numpy().real.copy()
+                vals = to_host(self.to_csc_device(t.to(self._dev.device)).real.contiguous())
         return csc_matrix((vals, self.indices, self.indptr), shape=(self.N, self.N))
 
 
@@ -137,4 +139,4 @@ def tet_mass_stiffness_matrices(field, er: np.ndarray, ur: np.ndarray,
     K, M = plan.assemble_km(er, ur, precision)
     if cscmap is None:
         cscmap = CSCMapping.from_plan(plan)
-    return K.cpu().numpy(), M.cpu().numpy(), cscmap
+    return to_host(K), to_host(M), cscmap

@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .assembly import _stream_ptr
+from .assembly import _stream_ptr, to_host
 
 
 def _vp(t):
@@ -187,14 +187,14 @@ class HeatCSCMapping:
     def __init__(self, dev: HeatCscMap):
         self._dev = dev
         ex = dev.export("indptr", "indices")
-        self.indptr = ex["indptr"].cpu().numpy()
-        self.indices = ex["indices"].cpu().numpy()
+        self.indptr = to_host(ex["indptr"])
+        self.indices = to_host(ex["indices"])
         self.N, self.nnz, self.nnz_coo = dev.N, dev.nnz, dev.nnz_coo
 
     def to_csc(self, data):
         import scipy.sparse as sp
         t = torch.as_tensor(np.ascontiguousarray(data, np.float64)).to(self._dev.device)
-        vals = self._dev.to_csc(t).cpu().numpy()
+        vals = to_host(self._dev.to_csc(t))
         return sp.csc_matrix((vals, self.indices, self.indptr), shape=(self.N, self.N))
 
 
@@ -213,11 +213,11 @@ def tet_stiffness_matrix(field, cond_termal):
     """grad.py:417-437: (conductance COO values float64 [100 T], CSCMapping)."""
     plan = _plan_for(field)
     K, _ = plan.assemble(cond=np.asarray(cond_termal, np.float64))
-    return K.cpu().numpy(), HeatCSCMapping(HeatCscMap(plan))
+    return to_host(K), HeatCSCMapping(HeatCscMap(plan))
 
 
 def tet_mass_matrix(field, rho_cp):
     """grad.py:440-462: (capacity COO values float64 [100 T], CSCMapping)."""
     plan = _plan_for(field)
     _, M = plan.assemble(rho_cp=np.asarray(rho_cp, np.float64))
-    return M.cpu().numpy(), HeatCSCMapping(HeatCscMap(plan))
+    return to_host(M), HeatCSCMapping(HeatCscMap(plan))

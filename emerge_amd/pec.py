@@ -27,7 +27,7 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .assembly import PREC_FP64, PREC_MIXED, Ned2Plan, _stream_ptr
+from .assembly import PREC_FP64, PREC_MIXED, Ned2Plan, _stream_ptr, to_host
 
 
 def _vp(t: torch.Tensor | None) -> ctypes.c_void_p | None:
@@ -181,5 +181,4 @@ def select_system(A, b, sel: SolveIds, drop_zeros: bool = False, stream=None):
 def to_scipy(indptr: torch.Tensor, indices: torch.Tensor, data: torch.Tensor, n: int):
     """SciPy ``csc_matrix`` of a device CSC triple (host copy)."""
     import scipy.sparse as sp
-    return sp.csc_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()),
-                         shape=(n, n))
+    return sp.csc_matrix((to_host(data), to_host(indices), to_host(indptr)), shape=(n, n))

@@ -322,6 +322,27 @@ def suite_dropin(args) -> dict:
         hK, hM = to_host(K), to_host(M)
     d2h_s = (time.perf_counter() - t0) / reps
     del hK, hM
+    # the reference's next step, cscmap.to_csc(Evec - Bvec * K0**2), on host COO values
+    k0 = 2 * np.pi * 10e9 / C0
+    coo = dE - dB * k0 ** 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A = cm.to_csc(coo)
+    to_csc_s = (time.perf_counter() - t0) / reps
+    nnz = A.nnz
+    del A
+    host = torch.from_numpy(coo)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d = host.to("cuda")
+        torch.cuda.synchronize()
+    h2d_pageable_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d = host.pin_memory().to("cuda", non_blocking=True)
+        torch.cuda.synchronize()
+    h2d_pinned_s = (time.perf_counter() - t0) / reps
+    del d, host, coo
     out_bytes = 2 * K.numel() * 16
     del K, M, dE, dB
     torch.cuda.empty_cache()
@@ -330,6 +351,8 @@ def suite_dropin(args) -> dict:
             "ms_device_assembly_km": dev_ms, "tets_per_s_device": T / (dev_ms * 1e-3),
             "s_d2h_copies": d2h_s, "d2h_gb_per_s": out_bytes / d2h_s / 1e9,
             "s_d2h_pageable": pageable_s, "d2h_pageable_gb_per_s": out_bytes / pageable_s / 1e9,
+            "s_to_csc_call": to_csc_s, "csc_nnz": nnz,
+            "s_h2d_coo_pageable": h2d_pageable_s, "s_h2d_coo_pinned_staged": h2d_pinned_s,
             "note": "fresh host arrays per call, as the reference returns; the drop-in "
                     "copies into pinned memory (s_d2h_copies), torch's .cpu() into "
                     "pageable memory (s_d2h_pageable)"}

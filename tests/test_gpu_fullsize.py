@@ -10,7 +10,9 @@ checksums) between the COO and CSC forms.
   mu_r, 10 GHz;
 * frequency sweep: 551k-tet mesh, 64 frequencies over 8-12 GHz in one CSC graph replay;
 * 32 meshes of 5k-400k tets (log-uniform), one launch, per-tet complex eps_r;
-* backward on a 1M-tet mesh: dL/d eps_r, dL/d mu_r^-1 for L = Re(lam^H A x).
+* backward on a 1M-tet mesh: dL/d eps_r, dL/d mu_r^-1 for L = Re(lam^H A x);
+* the 8M-tet target's CSC map beyond 2^31 entries (int64 indptr/indices): an 8.3M-tet
+  hex box with 53M DOFs.
 """
 import numpy as np
 import pytest
@@ -277,3 +279,77 @@ def test_backward_1M_tets():
     em = np.linalg.norm(fd_m - gm[sample, 2]) / np.linalg.norm(gm[sample, 2])
     print(f"1M tets: central differences vs backward: eps {ee:.2e}, mu^-1 {em:.2e}", flush=True)
     assert ee <= 1e-6 and em <= 1e-6
+
+
+def test_csc_map_beyond_2_31_entries():
+    """A CSC map of more than 2^31 entries, as the 8M-tet target has (csc_cast.py:65-160
+    with int64 indptr/indices): an 8,316,000-tet hex box (Kuhn split, 53,042,170 DOFs,
+    3.3e9 COO entries). Whole-map properties: indptr starts at 0, grows, ends at nnz >
+    2^31; every column's rows ascend strictly; to_csc of unit values gives every entry's
+    duplicate count, and the counts sum to the COO entries exactly. Sampled columns,
+    among them the last ones (stored past entry 2^31), equal the pattern built on the host
+    from the DOF map: the distinct rows of the incident tets and their multiplicities."""
+    import time
+    import torch
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    from emerge_amd.mesh import Nedelec2Field, hex_box_mesh
+    torch.cuda.empty_cache()
+    t0 = time.perf_counter()
+    f = Nedelec2Field(hex_box_mesh(1.0, 1.0, 1.0, 120, 110, 105))
+    T, N = f.n_tets, f.n_field
+    plan = Ned2Plan.from_field(f)
+    print(f"hex box: {T} tets, {N} DOFs ({time.perf_counter() - t0:.1f} s)", flush=True)
+    t0 = time.perf_counter()
+    cm = CscMap(plan)
+    torch.cuda.synchronize()
+    nnz, nnz_coo = cm.nnz, cm.nnz_coo
+    print(f"map: {nnz} entries, {nnz_coo} COO entries ({time.perf_counter() - t0:.2f} s)",
+          flush=True)
+    assert nnz > 2 ** 31 and nnz_coo == 400 * T
+    ex = cm.export("indptr", "indices")
+    indptr, indices = ex["indptr"], ex["indices"]
+    assert int(indptr[0]) == 0 and int(indptr[-1]) == nnz
+    assert bool((indptr[1:] > indptr[:-1]).all())  # every DOF is in some tet
+    starts = torch.zeros(nnz, dtype=torch.bool, device="cuda")
+    starts[indptr[:-1]] = True
+    step = 1 << 28
+    for a in range(0, nnz, step):
+        b = min(nnz, a + step)
+        blk = indices[a:b]
+        assert bool(((blk >= 0) & (blk < N)).all()), a
+        if b - a > 1:
+            inc = blk[1:] > blk[:-1]
+            assert bool((inc | starts[a + 1:b]).all()), a
+        if b < nnz:  # across the chunk boundary
+            assert bool(starts[b]) or int(indices[b]) > int(indices[b - 1]), b
+    del starts
+    ones = torch.ones(nnz_coo, dtype=torch.complex64, device="cuda")
+    counts = cm.to_csc(ones, block_symmetric=True)
+    del ones
+    total = 0
+    for a in range(0, nnz, step):
+        c = counts[a:a + step].real
+        assert bool((c >= 1).all()) and bool((c == torch.round(c)).all()), a
+        total += int(c.to(torch.int64).sum())
+    assert total == nnz_coo
+    # sampled columns against the host construction from the DOF map
+    t2f = np.asarray(f.tet_to_field)  # (20, T)
+    flat = t2f.T.ravel()  # entry 20 t + slot
+    order = np.argsort(flat, kind="stable")
+    first = np.searchsorted(flat[order], np.arange(N + 1))
+    rng = np.random.default_rng(17)
+    ip = indptr.cpu().numpy()
+    late = np.flatnonzero(ip[:-1] >= 2 ** 31)
+    cols = np.concatenate([rng.integers(0, N, 48), late[:8], late[-8:], [N - 1]])
+    for c in cols:
+        tets = order[first[c]:first[c + 1]] // 20
+        rows, mult = np.unique(t2f[:, tets].ravel(), return_counts=True)
+        a, b = int(ip[c]), int(ip[c + 1])
+        assert np.array_equal(indices[a:b].cpu().numpy(), rows), c
+        assert np.array_equal(counts[a:b].real.cpu().numpy().astype(np.int64), mult), c
+    print(f"{len(cols)} sampled columns equal the host construction "
+          f"({len(late)} columns stored past entry 2^31)", flush=True)
+    del counts, indices, indptr, ex
+    cm.close()
+    plan.close()
+    torch.cuda.empty_cache()

@@ -140,3 +140,24 @@ def test_reference_mode_km_and_sweep(wr90_field):
     Af = plan.assemble_a(er, ur, k0s, "fp64_ref").cpu().numpy()
     for i, k in enumerate(k0s):
         assert rel_fro(Af[i], dE - dB * k ** 2) <= 1e-13
+
+
+def test_dropin_reference_precision_with_conductors(delaunay_field):
+    """The drop-in's precision="fp64_ref" (curlcurl.py:222-270 signature, conductor tets
+    removed as :248-250 does) equals the verbatim oracle's dataE/dataB on the 21k-tet
+    Delaunay mesh with full complex tensors; other precisions are refused."""
+    from emerge_amd.curlcurl import tet_mass_stiffness_matrices
+    f = delaunay_field
+    T = f.n_tets
+    er, ur = _materials(T, 41, kind="full")
+    cond = np.arange(5, T, 97, dtype=np.int64)
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, er, ur, cond)
+    gE, gB, cm = tet_mass_stiffness_matrices(f, er, ur, cond, precision="fp64_ref")
+    assert gE.dtype == np.complex128 and gE.shape == dE.shape == (400 * (T - cond.size),)
+    assert rel_fro(gE, dE) <= 1e-14 and rel_fro(gB, dB) <= 1e-14
+    A = cm.to_csc(gE - K0 ** 2 * gB)
+    ref = oracle.CSCMap(rows, cols, f.n_field).to_csc(dE - K0 ** 2 * dB)
+    assert np.array_equal(A.indptr, ref.indptr) and np.array_equal(A.indices, ref.indices)
+    assert rel_fro(A.data, ref.data) <= 1e-14
+    with pytest.raises(ValueError):
+        tet_mass_stiffness_matrices(f, er, ur, cond, precision="mixed")

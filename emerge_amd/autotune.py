@@ -64,6 +64,24 @@ def tuning_key(plan, er, ur, nf: int, precision: str) -> str:
     return f"{arch}|{precision}|{mode}|eps{er.kind}|mu{ur.kind}|T2^{size}"
 
 
+def _time_cold(fn, reps: int, junk) -> float:
+    """Median of single launches, each after a fill of ``junk`` on the same stream (which
+    evicts the launch's inputs from the memory-side cache)."""
+    s = torch.cuda.current_stream()
+    fn()
+    ts = []
+    for _ in range(reps):
+        junk.fill_(0)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
 def _time(fn, reps: int, warmup: int) -> float:
     for _ in range(warmup):
         fn()
@@ -80,7 +98,7 @@ def _time(fn, reps: int, warmup: int) -> float:
 
 def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, warmup: int = 2,
          use_cache: bool = True, candidates=CANDIDATES, windows=WINDOWS,
-         rounds: int = 2) -> dict:
+         rounds: int = 2, cold: bool = False, flush_bytes: int = 1 << 30) -> dict:
     """Pick the fastest (window, tile) pair for ``plan.assemble_a(er, ur, k0, precision)``.
 
     Returns ``{"nb": tile, "window": window, "ms": {"window/tile": ms}, "cached": bool,
@@ -88,6 +106,11 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
     plan's window and tunes the tile only. Every candidate is timed in
     ``rounds`` interleaved passes and keeps its fastest pass: single trials of one
     candidate vary by a few percent, as much as the candidates differ.
+
+    ``cold=True`` times single launches that each follow a ``flush_bytes`` device fill, for
+    callers that run other work between assemblies: back-to-back launches of a mesh whose
+    inputs fit the 256 MB memory-side cache find them there, and the best window differs
+    (a 2.8M-tet mesh: 512-tet windows warm, 1,024 cold).
     """
     er_m, ur_m = plan.material(er), plan.material(ur)
     k0a = np.atleast_1d(np.asarray(k0, np.float64))
@@ -96,6 +119,8 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
         windows = (plan.get_window(),)
     else:
         key += "|win"
+    if cold:
+        key += "|cold"
     cache = _load() if use_cache else {}
     if use_cache and key in cache:
         c = cache[key]
@@ -109,15 +134,17 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
                           device=plan.device)
     k0arg = float(k0a[0]) if k0a.shape[0] == 1 else k0a
     target = out[0] if (k0a.shape[0] == 1 and out.dim() == 2) else out
+    junk = torch.empty(flush_bytes, dtype=torch.uint8, device=plan.device) if cold else None
     times = {}
     for _ in range(max(1, rounds)):
         for w in windows:
             plan.set_window(w)
             for nb in candidates:
                 plan.set_tuning(nb)
-                t = _time(lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target),
-                          reps, warmup)
+                fn = lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target)  # noqa: E731
+                t = _time_cold(fn, reps, junk) if cold else _time(fn, reps, warmup)
                 times[(w, nb)] = min(t, times.get((w, nb), t))
+    del junk
     best_w, best = min(times, key=times.get)
     plan.set_window(best_w)
     plan.set_tuning(best)

@@ -57,6 +57,32 @@ def test_autotuner_picks_and_caches(delaunay_field, tmp_path, monkeypatch):
     assert plan2.get_window() == r["window"]
 
 
+def test_autotuner_cold_cache_mode(delaunay_field, tmp_path, monkeypatch):
+    """cold=True times single launches after a device fill, keeps its own cache entry and
+    leaves the outputs unchanged."""
+    import torch
+    from emerge_amd import autotune
+    from emerge_amd.assembly import Ned2Plan
+    monkeypatch.setenv("EMERGE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    plan = Ned2Plan.from_field(delaunay_field)
+    eps, mu = _mats(plan.n_tets, np.random.default_rng(3))
+    k0 = 2 * np.pi * 10e9 / C0
+    ref = plan.assemble_a(eps, mu, k0, "mixed").cpu()
+    r = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, cold=True, flush_bytes=1 << 24,
+                      windows=(512, 1024), candidates=(64, 128), rounds=1)
+    assert r["key"].endswith("|cold") and not r["cached"]
+    assert set(r["ms"]) == {"512/64", "512/128", "1024/64", "1024/128"}
+    assert all(v > 0 for v in r["ms"].values())
+    assert plan.get_tuning() == r["nb"] and plan.get_window() == r["window"]
+    assert torch.equal(plan.assemble_a(eps, mu, k0, "mixed").cpu(), ref)
+    warm = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, windows=(512, 1024),
+                         candidates=(64, 128), rounds=1)
+    assert not warm["cached"] and warm["key"] != r["key"]
+    again = autotune.tune(plan, eps, mu, k0, "mixed", cold=True, windows=(512, 1024),
+                          candidates=(64, 128))
+    assert again["cached"] and again["nb"] == r["nb"] and again["window"] == r["window"]
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_window_variants_identical(delaunay_field, precision):
     """The window only reorders the tiles: K, M and A are the same bit for bit at every

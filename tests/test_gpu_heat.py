@@ -63,8 +63,8 @@ def test_heat_blocks_match_oracle(which, kind, wr90_field, delaunay_field):
 @pytest.mark.parametrize("every", [3, 1])
 def test_negative_orientation_tets(delaunay_field, every):
     """Tets stored with negative orientation follow the reference's |V| normalisation:
-    every third tet flipped (tiles of mixed orientation) and every tet flipped (tiles of
-    one negative orientation)."""
+    every third tet flipped (chunks of mixed orientation: threads switch coefficient sets
+    between chunks) and every tet flipped (every chunk negatively oriented)."""
     import torch
     from emerge_amd.heat import HeatPlan
     from emerge_amd.mesh import Legrange2Field, TetMesh
@@ -226,3 +226,35 @@ def test_heat_blocks_bit_symmetric_and_mirrored_gather(delaunay_field):
     ref = cm.to_csc(K, M, 250.0, segment_gather=True)
     assert torch.equal(cm.to_csc(K, M, 250.0, block_symmetric=True), ref)
     assert torch.equal(cm.to_csc(K, block_symmetric=True), cm.to_csc(K, segment_gather=True))
+
+
+@pytest.mark.parametrize("kind", ["diag", "full"])
+def test_heat_large_mesh_32_chunk_workgroups(kind):
+    """A mesh above 65,536 output chunks (~390k tets) runs 32 chunks per workgroup: a
+    strided sample of tets (every 37th, and the last) matches the oracle, with every fifth
+    tet negatively oriented so that threads switch coefficient sets between chunks, and M
+    alone (no conductivity) equals M of the joint call bit for bit."""
+    import torch
+    from emerge_amd.heat import HeatPlan
+    from emerge_amd.mesh import Legrange2Field, TetMesh, delaunay_mesh_with_tets
+    m0 = delaunay_mesh_with_tets(390_000, seed=77)
+    tets = m0.tets.copy()
+    flip = np.arange(0, m0.n_tets, 5)
+    tets[2, flip], tets[3, flip] = m0.tets[3, flip], m0.tets[2, flip]
+    f = Legrange2Field(TetMesh(m0.nodes, tets))
+    T = f.n_tets
+    assert T * 100 >= 65536 * 512, T
+    rng = np.random.default_rng(8)
+    C = _cond(T, rng, kind)
+    rho = rng.uniform(1, 2, T)
+    plan = HeatPlan.from_field(f)
+    K, M = plan.assemble(cond=torch.as_tensor(C), rho_cp=torch.as_tensor(rho))
+    _, M_only = plan.assemble(rho_cp=torch.as_tensor(rho))
+    assert torch.equal(M, M_only)
+    ids = np.unique(np.concatenate([np.arange(0, T, 37), [T - 1]]))
+    Kr, Mr, _, _ = oh.build(f.mesh.nodes, f.mesh.tets[:, ids], f.tet_to_field[:, ids],
+                            cond=C[:, :, ids], rho_cp=rho[ids], local_frame=True)
+    sel = (ids[:, None] * 100 + np.arange(100)[None, :]).ravel()
+    sel_t = torch.as_tensor(sel, device=K.device)
+    assert rel_fro(K[sel_t].cpu().numpy(), Kr) <= 1e-12
+    assert rel_fro(M[sel_t].cpu().numpy(), Mr) <= 1e-12

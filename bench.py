@@ -716,6 +716,79 @@ def suite_materials(args) -> dict:
                     "blocking copy of the count"}
 
 
+def suite_volume_step(args) -> dict:
+    """The whole volume step of assemble_freq_matrix (assembler.py:423-494) per frequency
+    on the bench mesh, through emerge_amd.assembler.VolumeAssembler with (3,3,T) complex128
+    device inputs (lossy diagonal er, diagonal ur, tand; no conductor tets): material
+    preparation, element matrices, CSC value array of A. Two material sets: conductivity 0
+    (nothing frequency dependent: K, M and their CSC sums cached after the first frequency,
+    then one combine per frequency) and a finite conductivity in 1% of the tets
+    (frequency dependent: A formed in the element kernel and gathered once per frequency;
+    beside it the K/M assembly plus fused combine that replaced it). Wall time per call,
+    synchronised, median of 5 frequencies after a first one."""
+    import numpy as np
+    import torch
+    from emerge_amd.assembler import VolumeAssembler
+    from emerge_amd.mesh import Nedelec2Field, delaunay_box_mesh
+    f = Nedelec2Field(delaunay_box_mesh(args.points, seed=args.seed))
+    T = f.n_tets
+    g = torch.Generator(device="cuda").manual_seed(args.seed + 31)
+    er = torch.zeros((3, 3, T), dtype=torch.complex128, device="cuda")
+    ur = torch.zeros_like(er)
+    tand = torch.zeros_like(er)
+    for d in range(3):
+        er[d, d] = torch.rand(T, generator=g, device="cuda", dtype=torch.float64) * 9 + 1
+        ur[d, d] = torch.rand(T, generator=g, device="cuda", dtype=torch.float64) + 1
+        tand[d, d] = torch.rand(T, generator=g, device="cuda", dtype=torch.float64) * 0.02
+    cond0 = torch.zeros_like(er)
+    cond1 = torch.zeros_like(er)
+    lossy = torch.arange(0, T, 100, device="cuda")
+    for d in range(3):
+        cond1[d, d, lossy] = 1e3
+    freqs = np.linspace(8e9, 12e9, 6)
+    res = {"n_tets": T}
+    for prec in ("fp64", "mixed"):
+        for name, cond in (("frequency_independent", cond0), ("frequency_dependent", cond1)):
+            va = VolumeAssembler(f, prec)
+            ts = []
+            for i, fr in enumerate(freqs):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = va.assemble_freq_matrix(er, ur, tand, cond, float(fr))
+                torch.cuda.synchronize()
+                if i:
+                    ts.append((time.perf_counter() - t0) * 1e3)
+            rec = {"ms_per_frequency": float(np.median(ts)), "nnz": int(r.data.numel()),
+                   "is_frequency_dependent": r.is_frequency_dependent,
+                   "assemblies": va.assemblies}
+            if name == "frequency_dependent":
+                # the K/M path it replaced: both COO arrays, then the fused combine
+                plan, er_m, ur_m, _, _ = va._prepare(er, ur, tand, cond, float(freqs[-1]))
+                cm = va._cscmap(plan)
+                k0 = 2 * np.pi * freqs[-1] / C0
+
+                def km_path():
+                    K, M = plan.assemble_km(er_m, ur_m, prec)
+                    return cm.to_csc_combine(K, M, k0, block_symmetric=True)
+                km_path()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    ref = km_path()
+                torch.cuda.synchronize()
+                rec["ms_km_path_without_prep"] = (time.perf_counter() - t0) * 1e3 / 3
+                a = va.assemble_freq_matrix(er, ur, tand, cond, float(freqs[-1])).data
+                rec["rel_diff_vs_km_path"] = float(
+                    torch.linalg.norm((a - ref).to(torch.complex128)) /
+                    torch.linalg.norm(ref.to(torch.complex128)))
+                del ref, a
+            res[f"{prec}_{name}"] = rec
+            log(f"volume step {prec} {name}: {rec}")
+            del va, r
+            torch.cuda.empty_cache()
+    return res
+
+
 def suite_heat(args) -> dict:
     """Lagrange-2 heat assembly (conductance and capacity, FP64 COO) on a ~2.8M-tet mesh
     (the bench mesh), its CSC map and the per-step to_csc(K + M / dt); CPU line: the
@@ -1013,6 +1086,7 @@ def main() -> int:
                     ("surface_interp", "surface_interp_500k", suite_surface_interp),
                     ("heat", "heat_2.8M", suite_heat),
                     ("materials", "materials_2.8M", suite_materials),
+                    ("volume", "volume_step_2.8M", suite_volume_step),
                     ("ref", "fp64_ref_500k", suite_ref),
                     ("dropin", "dropin_500k", suite_dropin))
             only = set(args.only.split(",")) if args.only else None

@@ -9,9 +9,12 @@ converts K and M separately instead of combining them):
    dependence and the conductor tets (:439-469) -- ``em_material_prep``;
 3. the element matrices without the conductor tets (:479-485), cached as the reference's
    ``cached_matrices`` and reused while nothing is frequency dependent (:471-478);
-4. ``cached_cscmap.to_csc(Evec - Bvec * K0**2)`` (:494) -- the fused CSC combine, or,
-   while the element matrices are reused, ``K_csc - K0**2 M_csc`` from CSC sums of K and M
-   kept with them (``FreqCsc``: the same matrix up to the order of each entry's sum).
+4. ``cached_cscmap.to_csc(Evec - Bvec * K0**2)`` (:494) -- while the element matrices
+   are reused, ``K_csc - K0**2 M_csc`` from CSC sums of K and M kept with them
+   (``FreqCsc``: the same matrix up to the order of each entry's sum); with
+   frequency-dependent materials the element kernel forms ``E - K0**2 B`` itself and one
+   gather converts it (the same matrix up to rounding: ``-K0**2`` is folded into the mass
+   metric tensor); with caching off, the fused CSC combine of K and M.
 
 Everything stays on the GPU; the result is the CSC value array of ``A`` with the map's
 ``indptr``/``indices``. The cached matrices and CSC map belong to the conductor set they
@@ -28,26 +31,21 @@ import numpy as np
 import torch
 
 from . import _lib
-from .assembly import CscMap, FreqCsc, Material
+from .assembly import CscMap, FreqCsc, Material, ref_structure
 from .curlcurl import plan_for
 from .materials import PECLIM, SURFIMPLIM, prepare_materials
 
 C0 = 299792458  # const.py:20
 
-_OFF = ([0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1])
-
-
 def _prepared_kind(er, tand, cond) -> int:
-    """Structure of er(1 - j tand) - j cond/(w eps0) known from host inputs: diagonal when
-    er and cond are (an off-diagonal er of 0 stays 0 whatever tand is), scalar when every
-    input's three diagonal entries are equal. Device inputs are taken as full."""
-    if any(isinstance(x, torch.Tensor) for x in (er, tand, cond)):
+    """Structure of er(1 - j tand) - j cond/(w eps0): diagonal when er and cond are (an
+    off-diagonal er of 0 stays 0 whatever tand is), scalar when every input's three
+    diagonal entries are equal. Checked where each input lives (host or device)."""
+    s = [ref_structure(x if isinstance(x, torch.Tensor) else np.asarray(x))
+         for x in (er, tand, cond)]
+    if not (s[0][0] and s[2][0]):
         return _lib.EM_MAT_FULL
-    er, tand, cond = (np.asarray(x) for x in (er, tand, cond))
-    if np.any(er[_OFF]) or np.any(cond[_OFF]):
-        return _lib.EM_MAT_FULL
-    if all(np.array_equal(a[0, 0], a[1, 1]) and np.array_equal(a[0, 0], a[2, 2])
-           for a in (er, tand, cond)):
+    if all(d for _, d in s):
         return _lib.EM_MAT_SCALAR
     return _lib.EM_MAT_DIAG
 
@@ -96,20 +94,35 @@ class VolumeAssembler:
         materials' own ``frequency_dependent`` flags (:433-436)."""
         W0 = 2 * np.pi * frequency
         K0 = W0 / C0
-        plan, er_m, ur_m, conductor_tets, fd = self._prepare(er, ur, tand, cond, frequency)
+        plan, er_d, conductor_tets, fd = self._prepare_er(er, tand, cond, frequency)
         fd = fd or frequency_dependent
         c = self.cached_matrices
-        if cache_matrices and not fd and c is not None and c[0] is plan:
+        if fd:
+            er_m, ur_m = self._descriptors(plan, er_d, er, ur, tand, cond)
+            # frequency-dependent materials: nothing can be reused, so A = K - K0^2 M is
+            # formed inside the element kernel (one COO array written and gathered instead
+            # of two) and converted once. Nothing is cached: the reference keeps this
+            # call's E and B, which only a later call with frequency-independent materials
+            # (built from other materials) would reuse; such a call assembles afresh here.
+            sym = _symmetric(er_m, ur_m)
+            A = plan.assemble_a(er_m, ur_m, K0, self.precision)
+            self.assemblies += 1
+            self.cached_matrices = None
+            self._freq = None
+            cm = self._cscmap(plan)
+            return FreqMatrix(cm.to_csc(A, block_symmetric=sym), cm, conductor_tets, fd, K0)
+        if cache_matrices and c is not None and c[0] is plan:
             # the symmetry of the cached blocks is that of the materials they were built
             # from, not of this call's
             K, M, sym = c[1], c[2], c[3]
         else:
+            er_m, ur_m = self._descriptors(plan, er_d, er, ur, tand, cond)
             K, M = plan.assemble_km(er_m, ur_m, self.precision)
             sym = _symmetric(er_m, ur_m)
             self.assemblies += 1
             self.cached_matrices = (plan, K, M, sym)
         cm = self._cscmap(plan)
-        if cache_matrices and not fd:
+        if cache_matrices:
             # while K and M are reused, so are their CSC sums: each frequency is then one
             # streaming pass A = K_csc - K0^2 M_csc instead of a gather over the COO values
             if self._freq is None or self._freq[0] is not K:
@@ -148,15 +161,27 @@ class VolumeAssembler:
     def _prepare(self, er, ur, tand, cond, frequency):
         """Prepared materials on the device, the plan of their conductor set and the
         kernels' descriptors of er' and ur."""
+        plan, er_d, conductor_tets, fd = self._prepare_er(er, tand, cond, frequency)
+        er_m, ur_m = self._descriptors(plan, er_d, er, ur, tand, cond)
+        return plan, er_m, ur_m, conductor_tets, fd
+
+    def _prepare_er(self, er, tand, cond, frequency):
+        """er' = er(1 - j tand) - j cond/(w eps0) on the device, the plan of the conductor
+        set and the frequency dependence."""
         er_d, conductor_tets, fd = prepare_materials(er, tand, cond, frequency, self.peclim,
                                                      self.surfimplim)
-        plan = plan_for(self.field, conductor_tets)
+        return plan_for(self.field, conductor_tets), er_d, conductor_tets, fd
+
+    def _descriptors(self, plan, er_d, er, ur, tand, cond):
+        """The kernels' descriptors of er' and ur, with their structure (scalar, diagonal
+        or full) checked where the inputs live; only needed when element matrices are
+        assembled, not when cached ones are reused."""
         T = plan.n_tets
         kind = _prepared_kind(er, tand, cond)
         comp = {_lib.EM_MAT_FULL: T, _lib.EM_MAT_DIAG: 4 * T, _lib.EM_MAT_SCALAR: 0}[kind]
         # ur is a (3, 3, T) tensor whether it lives on the host or on the device
         ur_m = plan.material(ur, layout="ref")
-        return plan, Material(er_d, kind, 1, comp), ur_m, conductor_tets, fd
+        return Material(er_d, kind, 1, comp), ur_m
 
     def _cscmap(self, plan) -> CscMap:
         """The CSC map of ``plan`` (the reference's ``cached_cscmap``)."""

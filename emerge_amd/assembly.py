@@ -76,6 +76,22 @@ class Material:
                              self.tet_stride, self.comp_stride)
 
 
+_OFF_DIAG = ([0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1])
+
+
+def ref_structure(a) -> tuple[bool, bool]:
+    """For a (3, 3, T) array (NumPy, or a torch tensor on any device, checked where it
+    lives): (every off-diagonal entry is zero, the three diagonal entries are equal)."""
+    if isinstance(a, torch.Tensor):
+        # per-component views: no copy of the tensor, one synchronisation per question
+        nz = torch.stack([(a[p, q] != 0).any() for p, q in zip(*_OFF_DIAG)])
+        off_zero = not bool(nz.any())
+        same = torch.stack([(a[0, 0] != a[1, 1]).any(), (a[0, 0] != a[2, 2]).any()])
+        return off_zero, not bool(same.any())
+    off_zero = not any(np.any(a[p, q]) for p, q in zip(*_OFF_DIAG))
+    return off_zero, bool(np.array_equal(a[0, 0], a[1, 1]) and np.array_equal(a[0, 0], a[2, 2]))
+
+
 def material(value, n_tets: int, device, inverse: bool = False, detect: bool = True,
              layout: str | None = None) -> Material:
     """Describe a per-tet material tensor field for the kernels.
@@ -84,7 +100,8 @@ def material(value, n_tets: int, device, inverse: bool = False, detect: bool = T
     ``"tet"`` is tet-major: ``(T,)`` scalar per tet, ``(T, 3)`` diagonal, ``(T, 3, 3)``
     full. A Python/NumPy scalar is one value for every tet. By default NumPy arrays of
     shape ``(3, 3, T)`` are ``"ref"`` and everything else is ``"tet"``.
-    For a ``"ref"`` host array with ``detect=True`` the structure is checked on the host
+    For a ``"ref"`` array with ``detect=True`` the structure is checked (on the host for a
+    NumPy array, on the device for a tensor there: one reduction and a synchronisation)
     and a diagonal or scalar field is described as such (its diagonal is read in place).
     ``inverse`` marks a mu field that already holds mu_r^-1.
     """
@@ -105,12 +122,10 @@ def material(value, n_tets: int, device, inverse: bool = False, detect: bool = T
         if shape != (3, 3, n_tets):
             raise ValueError(f"reference-layout material must be (3, 3, {n_tets}), got {shape}")
         kind = _lib.EM_MAT_FULL
-        if detect and not is_torch:
-            off = arr[[0, 0, 1, 1, 2, 2], [1, 2, 0, 2, 0, 1], :]
-            if not np.any(off):
-                kind = _lib.EM_MAT_DIAG
-                if np.array_equal(arr[0, 0], arr[1, 1]) and np.array_equal(arr[0, 0], arr[2, 2]):
-                    kind = _lib.EM_MAT_SCALAR
+        if detect:
+            off_zero, diag_equal = ref_structure(t if is_torch else arr)
+            if off_zero:
+                kind = _lib.EM_MAT_SCALAR if diag_equal else _lib.EM_MAT_DIAG
         if kind == _lib.EM_MAT_FULL:
             return Material(t, kind, 1, n_tets, flags)
         if kind == _lib.EM_MAT_DIAG:
@@ -283,7 +298,7 @@ class Ned2Plan:
         if out is None:
             out = torch.empty((nf, self.nnz_coo), dtype=dt, device=self.device)
         else:
-            _check_out(out, nf * self.nnz_coo, dt)
+            _check_out(out, nf * self.nnz_coo, dt, rows=nf)
         stride = self.nnz_coo if out.dim() == 1 else out.stride(0)
         e, u = er_m.c_struct(), ur_m.c_struct()
         check(_lib.lib().em_ned2_assemble_a(self._h, ctypes.byref(e), ctypes.byref(u),
@@ -475,7 +490,7 @@ def csc_freq_combine(K: torch.Tensor, M: torch.Tensor, k0, out=None,
     if out is None:
         out = torch.empty((nf, n), dtype=K.dtype, device=K.device)
     else:
-        _check_out(out, nf * n, K.dtype)
+        _check_out(out, nf * n, K.dtype, rows=nf)
     stride = n if out.dim() == 1 else out.stride(0)
     check(_lib.lib().em_csc_freq_combine(n, ctypes.c_void_p(K.data_ptr()),
                                          ctypes.c_void_p(M.data_ptr()),
@@ -513,10 +528,22 @@ class FreqCsc:
         return csc_freq_combine(self.K, self.M, k0, out=out, stream=stream)
 
 
-def _check_out(t: torch.Tensor, numel: int, dt) -> None:
+def _check_out(t: torch.Tensor, numel: int, dt, rows: int = 1) -> None:
+    """An output of ``numel`` values in ``rows`` rows (one per wavenumber): a contiguous
+    1-D tensor, or a 2-D tensor of shape ``(rows, numel // rows)`` whose rows are
+    contiguous and do not overlap (row stride at least the row length)."""
     if t.dtype != dt or not t.is_cuda or t.numel() != numel:
         raise ValueError(f"output must be a {dt} CUDA tensor with {numel} elements")
-    if t.dim() == 1 and not t.is_contiguous():
-        raise ValueError("output must be contiguous")
-    if t.dim() == 2 and t.stride(1) != 1:
-        raise ValueError("output rows must be contiguous")
+    if t.dim() == 1:
+        if not t.is_contiguous():
+            raise ValueError("output must be contiguous")
+    elif t.dim() == 2:
+        n = numel // rows
+        if tuple(t.shape) != (rows, n):
+            raise ValueError(f"2-D output must have shape ({rows}, {n})")
+        if n > 1 and t.stride(1) != 1:
+            raise ValueError("output rows must be contiguous")
+        if rows > 1 and t.stride(0) < n:
+            raise ValueError("output rows must not overlap")
+    else:
+        raise ValueError("output must be 1-D or 2-D")

@@ -110,6 +110,11 @@ def test_fused_a_single_and_sweep(delaunay_field):
     Af = plan.assemble_a(er, ur, k0s, "fp64").cpu().numpy()
     for f, k0 in enumerate(k0s):
         assert rel_fro(Af[f], dE - dB * k0 ** 2) <= FP64_TOL
+    import torch
+    # a sweep's output needs one row per wavenumber (a single row would be overrun)
+    with pytest.raises(ValueError):
+        plan.assemble_a(er, ur, k0s, "fp64", out=torch.empty(
+            (1, len(k0s) * plan.nnz_coo), dtype=torch.complex128, device=plan.device))
 
 
 @pytest.mark.parametrize("eps_kind,mu_kind", [("scalar", "diag"), ("full", "sym")])

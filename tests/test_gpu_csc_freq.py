@@ -67,6 +67,13 @@ def test_csc_freq_combine_edges(dtype):
         assert torch.isnan(out[:, n:].real).all()  # padding untouched
     empty = torch.empty(0, dtype=dt, device="cuda")
     assert csc_freq_combine(empty, empty, [1.0, 2.0]).shape == (2, 0)
+    # a multi-frequency output must have one row per wavenumber
+    K = torch.randn(8, dtype=dt, device="cuda", generator=g)
+    for bad in (torch.empty((1, 16), dtype=dt, device="cuda"),
+                torch.empty((4, 4), dtype=dt, device="cuda"),
+                torch.empty((2, 2, 4), dtype=dt, device="cuda")):
+        with pytest.raises(ValueError):
+            csc_freq_combine(K, K, [1.0, 2.0], out=bad)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

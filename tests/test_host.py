@@ -274,7 +274,8 @@ def test_prepared_material_structure_matches_the_arithmetic():
         if kind == _lib.EM_MAT_DIAG:
             assert not (np.array_equal(out[0, 0], out[1, 1]) and
                         np.array_equal(out[0, 0], out[2, 2])), name
-    # tensors already on a device are not inspected: described as full
+    # torch tensors are inspected where they live (here on the host): same answers
     import torch
-    er, tand, cond = cases["scalar"]
-    assert _prepared_kind(torch.from_numpy(er), tand, cond) == _lib.EM_MAT_FULL
+    for name, (er, tand, cond) in cases.items():
+        assert _prepared_kind(torch.from_numpy(er), torch.from_numpy(tand), cond) == \
+            want[name], name

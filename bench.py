@@ -186,13 +186,28 @@ def csc_per_frequency(plan, eps, mu, k0, steps: int, warmup: int) -> dict:
         ms_gather, _ = time_launches(
             lambda: cm.to_csc_combine(K, M, k0, out=out, block_symmetric=True),
             max(2, steps // 2), 1)
+        # A straight into CSC, without the COO arrays (em_ned2_assemble_csc), beside the
+        # two-step assemble_a + to_csc it equals bit for bit
+        ms_direct, _ = time_launches(
+            lambda: plan.assemble_csc(cm, eps, mu, k0, "A", prec, out=out),
+            max(2, steps // 2), 1)
+        a_coo = plan.assemble_a(eps, mu, k0, prec)
+        ms_two_step, _ = time_launches(
+            lambda: cm.to_csc(plan.assemble_a(eps, mu, k0, prec, out=a_coo), out=out,
+                              block_symmetric=True), max(2, steps // 2), 1)
+        direct_equal = bool(torch.equal(plan.assemble_csc(cm, eps, mu, k0, "A", prec),
+                                        cm.to_csc(a_coo, block_symmetric=True)))
+        del a_coo
         b = 3 * cm.nnz * c
         res[prec] = {"ms_per_frequency": ms, "bytes_per_frequency": b,
                      "gb_per_s": b / (ms * 1e-3) / 1e9,
                      "roofline_frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "ms_one_time_csc_sums_of_K_and_M": sums_ms,
                      "ms_per_frequency_fused_coo_gather": ms_gather,
-                     "rel_diff_vs_fused_gather": diff}
+                     "rel_diff_vs_fused_gather": diff,
+                     "ms_direct_csc_A_no_coo": ms_direct,
+                     "ms_assemble_a_then_to_csc": ms_two_step,
+                     "direct_equals_two_step": direct_equal}
         log(f"csc per frequency ({prec}): {ms:.3f} ms combine, {ms_gather:.3f} ms fused gather")
         del K, M, fc, out
         torch.cuda.empty_cache()

@@ -309,6 +309,29 @@ class Ned2Plan:
             return out[0]
         return out
 
+    def assemble_csc(self, cscmap, er, ur, k0: float = 0.0, which: str = "A",
+                     precision="fp64", out=None, stream=None) -> torch.Tensor:
+        """CSC values (in ``cscmap``'s pattern) of ``A = K - k0^2 M`` (``which="A"``), of K
+        (``"K"``) or of M (``"M"``), assembled without the COO arrays
+        (``em_ned2_assemble_csc``); equal bit for bit to ``to_csc`` of ``assemble_a`` /
+        ``assemble_km``. ``cscmap`` is a map of this plan or restricted from one."""
+        prec = _prec(precision)
+        if prec == _lib.EM_PREC_FP64_REF:
+            raise ValueError("assemble_csc takes 'fp64' or 'mixed'")
+        w = {"A": 0, "K": 1, "M": 2}[which]
+        er_m, ur_m = self.material(er), self.material(ur)
+        dt = _out_dtype(prec)
+        if out is None:
+            out = torch.empty(cscmap.nnz, dtype=dt, device=self.device)
+        else:
+            _check_out(out, cscmap.nnz, dt)
+        e, u = er_m.c_struct(), ur_m.c_struct()
+        check(_lib.lib().em_ned2_assemble_csc(self._h, cscmap._h, ctypes.byref(e),
+                                              ctypes.byref(u), float(k0), w,
+                                              ctypes.c_void_p(out.data_ptr()), prec,
+                                              _stream_ptr(stream)))
+        return out
+
     def backward(self, grad_coo: torch.Tensor, k0: float, eps_kind: int | None, grad_eps,
                  mu_kind: int | None, grad_mu_inv, stream=None) -> None:
         """dL/d eps_r and dL/d mu_r^-1 (tet-major complex128 buffers, written in place)

@@ -37,6 +37,8 @@ struct em_ned2_plan_s {
     int tile = 64;  // tets per workgroup tile of the assembly kernel (autotuned)
     long long window = 1024;        // tets per code-grouping window (autotuned)
     std::vector<unsigned char> code;  // vertex-rank code of every assembly index (host)
+    void* csc_rec = nullptr;        // per-tet records of the direct CSC assembly
+    size_t csc_rec_bytes = 0;
     int* order = nullptr;           // assembly indices grouped by vertex-rank code
     int2* tiles[3] = {nullptr, nullptr, nullptr};
     long long ntiles[3] = {0, 0, 0};
@@ -103,6 +105,7 @@ void free_plan(em_ned2_plan_s* p) {
     (void)hipFree(p->tetids);
     (void)hipFree(p->order);
     for (int i = 0; i < 3; ++i) (void)hipFree(p->tiles[i]);
+    (void)hipFree(p->csc_rec);
     delete p;
 }
 
@@ -822,6 +825,64 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
 int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                      void* data_csc, int32_t precision, int32_t flags, void* stream) {
     return csc_apply(map, dataE, dataB, k0, data_csc, precision, flags, stream);
+}
+
+int em_ned2_assemble_csc(em_ned2_plan_t plan, em_csc_map_t map, const em_material_t* er,
+                         const em_material_t* ur, double k0, int32_t which, void* data_csc,
+                         int32_t precision, void* stream) {
+    if (!plan || !map) {
+        set_error("assemble_csc: null plan or map");
+        return EM_EINVAL;
+    }
+    int st;
+    if ((st = check_material(er, plan->n_tets, "er")) || (st = check_material(ur, plan->n_tets, "ur")))
+        return st;
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("assemble_csc: precision must be FP64 or MIXED");
+        return EM_EINVAL;
+    }
+    if (which < 0 || which > 2) {
+        set_error("assemble_csc: which must be 0 (A), 1 (K) or 2 (M)");
+        return EM_EINVAL;
+    }
+    if (!std::isfinite(k0)) {
+        set_error("assemble_csc: non-finite k0");
+        return EM_EINVAL;
+    }
+    if (map->b.ndof != 20 || !map->b.inc || !map->b.pos || map->device != plan->device ||
+        map->nnz_coo != plan->n_assy * 400) {
+        set_error("assemble_csc: the map must be a Nedelec map of this plan (or restricted "
+                  "from one)");
+        return EM_EINVAL;
+    }
+    if (map->b.nnz > 0 && !data_csc) {
+        set_error("null output");
+        return EM_EINVAL;
+    }
+    if ((uintptr_t)data_csc & 15) {
+        set_error("output must be 16-byte aligned");
+        return EM_EINVAL;
+    }
+    DeviceGuard g(plan->device);
+    const int full = needs_anti(er, ur);
+    const size_t need = em::assemble_csc_record_bytes(plan->n_assy, precision, full);
+    if (need > plan->csc_rec_bytes) {
+        // launches still queued may read the old records: hipFree waits for the device
+        (void)hipFree(plan->csc_rec);
+        plan->csc_rec = nullptr;
+        plan->csc_rec_bytes = 0;
+        const hipError_t e = hipMalloc(&plan->csc_rec, need);
+        if (e != hipSuccess) {
+            plan->csc_rec = nullptr;
+            set_error(std::string("assemble_csc: records: ") + hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? EM_ENOMEM : EM_EHIP;
+        }
+        plan->csc_rec_bytes = need;
+    }
+    EM_HIP(em::launch_assemble_csc(plan->dev(), map->b, map->N, to_desc(er, false),
+                                   to_desc(ur, true), which, precision, full, k0 * k0,
+                                   plan->csc_rec, data_csc, (hipStream_t)stream));
+    return EM_OK;
 }
 
 int em_csc_to_csc_staged(em_csc_map_t map, const void* dataE, const void* dataB, double k0,

@@ -106,6 +106,13 @@ hipError_t csc_build(const MeshDev& m, long long N, CscBuild& b, long long& max_
 hipError_t csc_build_ndof10(const int* t2f, long long n_tets, long long N, CscBuild& b,
                             long long& max_inc, hipStream_t s);
 // out[q] = sum of E[k] (+ scale * B[k]) over the contributors of entry q (real data).
+// Direct CSC assembly (ned2_assemble.hip): per-tet records into `rec` (at least
+// assemble_csc_record_bytes), then the CSC values of A (which 0, G_M scaled by -k0sq),
+// K (1) or M (2) for the columns of map b.
+size_t assemble_csc_record_bytes(long long n_assy, int precision, int full);
+hipError_t launch_assemble_csc(const MeshDev& m, const CscBuild& b, long long N,
+                               const MatDesc& eps, const MatDesc& mu, int which, int precision,
+                               int full, double k0sq, void* rec, void* out, hipStream_t s);
 // Real-valued column-incidence gather (10-DOF maps; needs b.inc and b.pos).
 hipError_t csc_gather_inc_real(const CscBuild& b, long long N, const double* E, const double* B,
                                double scale, bool mirror, double* out, hipStream_t s);

@@ -304,6 +304,49 @@ __device__ __forceinline__ void store2(double2* blk, int e0, int e1, double v0r,
 // parts of every K / M metric tensor of the tile are zero (a real mu_r / eps_r), so their
 // products (all +-0, which leave a +0 accumulator at +0) are skipped; the result is the
 // same bit for bit.
+// A's entry from the K and M parts of one slot (G_M carries -k0^2), added without
+// contraction so that every kernel forming it rounds the same way.
+template <typename R>
+__device__ __forceinline__ R add_km(R k, R m) {
+#pragma clang fp contract(off)
+    return k + m;
+}
+
+// K and M entries of one output slot of one tet: the slot's canonical-pair coefficients
+// (tk, tm; qk, qm for the antisymmetric parts) contracted with the tet's metric tensors,
+// times sg (the orientation sign when the pair joins a code-64 and an odd-degree
+// function, else 1). KRE / MRE: the imaginary parts are known to be zero and their
+// products are skipped (they are +-0 and leave a +0 accumulator at +0: same bits).
+template <typename R, bool FULL, bool KRE, bool MRE>
+__device__ __forceinline__ void pair_entry(const R (&tk)[6], const R (&tm)[6], const R (&qk)[3],
+                                           const R (&qm)[3], const R (&gkr)[6],
+                                           const R (&gki)[6], const R (&gmr)[6],
+                                           const R (&gmi)[6], const R (*ak)[2],
+                                           const R (*am)[2], R sg, R& kr, R& ki, R& mr,
+                                           R& mi) {
+    R a = 0, b = 0, c2 = 0, d = 0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        a += tk[c] * gkr[c];
+        if constexpr (!KRE) b += tk[c] * gki[c];
+        c2 += tm[c] * gmr[c];
+        if constexpr (!MRE) d += tm[c] * gmi[c];
+    }
+    if constexpr (FULL) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a += qk[c] * ak[c][0];
+            if constexpr (!KRE) b += qk[c] * ak[c][1];
+            c2 += qm[c] * am[c][0];
+            if constexpr (!MRE) d += qm[c] * am[c][1];
+        }
+    }
+    kr = sg * a;
+    ki = sg * b;
+    mr = sg * c2;
+    mi = sg * d;
+}
+
 template <typename R, int MODE, bool FULL, int TILE, bool KRE, bool MRE>
 __device__ __forceinline__ void stream_tile(const StreamRec<R, FULL, TILE>& rec, int cnt,
                                             const int (&E)[2], const R (&tk)[2][6],
@@ -325,34 +368,14 @@ __device__ __forceinline__ void stream_tile(const StreamRec<R, FULL, TILE>& rec,
         const long long base = (long long)rec.ia[lt] * 400;
         R kr[2], ki[2], mr[2], mi[2];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            R a = 0, b = 0, c2 = 0, d = 0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                a += tk[q][c] * gkr[c];
-                if constexpr (!KRE) b += tk[q][c] * gki[c];
-                c2 += tm[q][c] * gmr[c];
-                if constexpr (!MRE) d += tm[q][c] * gmi[c];
-            }
-            if constexpr (FULL) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    a += qk[q][c] * rec.ak[lt][c][0];
-                    if constexpr (!KRE) b += qk[q][c] * rec.ak[lt][c][1];
-                    c2 += qm[q][c] * rec.am[lt][c][0];
-                    if constexpr (!MRE) d += qm[q][c] * rec.am[lt][c][1];
-                }
-            }
-            const R sg = odd[q] ? s : (R)1;
-            kr[q] = sg * a;
-            ki[q] = sg * b;
-            mr[q] = sg * c2;
-            mi[q] = sg * d;
-        }
+        for (int q = 0; q < 2; ++q)
+            pair_entry<R, FULL, KRE, MRE>(tk[q], tm[q], qk[q], qm[q], gkr, gki, gmr, gmi,
+                                          rec.ak[FULL ? lt : 0], rec.am[FULL ? lt : 0],
+                                          odd[q] ? s : (R)1, kr[q], ki[q], mr[q], mi[q]);
         if (MODE == MODE_A) {
             // G_M already carries -k0^2
-            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0] + mr[0], ki[0] + mi[0],
-                   kr[1] + mr[1], ki[1] + mi[1]);
+            store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], add_km(kr[0], mr[0]),
+                   add_km(ki[0], mi[0]), add_km(kr[1], mr[1]), add_km(ki[1], mi[1]));
         } else if (MODE == MODE_KM) {
             store2(reinterpret_cast<O*>(out0) + base, E[0], E[1], kr[0], ki[0], kr[1], ki[1]);
             store2(reinterpret_cast<O*>(out1) + base, E[0], E[1], mr[0], mi[0], mr[1], mi[1]);
@@ -364,6 +387,26 @@ __device__ __forceinline__ void stream_tile(const StreamRec<R, FULL, TILE>& rec,
                        ki[1] - kf * mi[1]);
             }
         }
+    }
+}
+
+// Coefficients of slot (ii, jj) alone (as slot_coefs for one entry).
+template <typename R, bool FULL, typename Tab>
+__device__ __forceinline__ void slot_coefs1(const Tab& tab, int code, int ii, int jj,
+                                            R (&tk)[6], R (&tm)[6], R (&qk)[3], R (&qm)[3]) {
+    const int ca = tab.canon(code, ii), cb = tab.canon(code, jj);
+    const int p = tab.pair(ca, cb);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        tk[c] = (R)tab.sk(p, c);
+        tm[c] = (R)tab.sm(p, c);
+    }
+    const int slot_lo = (ca <= cb) ? ii : jj, slot_hi = (ca <= cb) ? jj : ii;
+    const R sa = (slot_lo < slot_hi) ? (R)-1 : (R)1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        qk[c] = FULL ? sa * (R)tab.ak(p, c) : (R)0;
+        qm[c] = FULL ? sa * (R)tab.am(p, c) : (R)0;
     }
 }
 
@@ -528,6 +571,278 @@ hipError_t launch_assemble(const MeshDev& m, const TileSet* ts, int tile, const 
                                    freq_stride, stream);
     return launch_mode<float>(m, ts, tile, eps, mu, mode, full, k0sq, nf, out0, out1, freq_stride,
                               stream);
+}
+
+// ---------------------------------------------------------------------------
+// Direct CSC assembly: the CSC values of A (or of K, or of M) without the COO arrays.
+//
+// The reference converts the COO blocks with to_csc (csc_cast.py:55-63, assembler.py:494);
+// the column-incidence gather of csc_map.hip reads 8-16 B per contributor that the
+// assembly kernel wrote just before. Here the contributors are computed where they are
+// summed: a first kernel writes each tet's metric tensors, orientation sign and
+// vertex-rank code (kRecLen values of R, 128-320 B), a second walks the columns as the
+// incidence gather does (three columns per wavefront, one group of 20 lanes each; lane ii
+// takes row slot ii of every incident (tet, slot jj) in incidence order = ascending COO
+// index) and computes entry (ii, jj) of the tet's block from its record and the
+// reference tensors (an LDS copy), with the assembly kernel's arithmetic (pair_entry), so
+// the values equal the COO blocks' bit for bit and their FP64 sums equal to_csc of those
+// blocks bit for bit. The block entry is computed directly, so symmetric and
+// non-symmetric blocks take the same path. Per incidence: 4 B of incidence, 40 B of row
+// positions and one record (mostly from L2: the 20 columns of a tet's DOFs read it)
+// instead of 20 contributors.
+// ---------------------------------------------------------------------------
+template <bool FULL>
+struct RecLayout {
+    static constexpr int kLen = FULL ? 40 : 32;  // values of R per tet (padded)
+    static constexpr int kGk = 0, kGm = 12, kAk = 24, kAm = 30;
+    static constexpr int kSgn = kLen - 2, kCode = kLen - 1;
+};
+
+template <typename R, bool FULL>
+__global__ __launch_bounds__(256) void ned2_metric_kernel(MeshDev m, MatDesc eps, MatDesc mu,
+                                                          double mscale, R* __restrict__ rec) {
+    using L = RecLayout<FULL>;
+    const long long ia = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (ia >= m.n_assy) return;
+    const long long t = m.tetids ? (long long)m.tetids[ia] : ia;
+    R v[L::kLen];
+#pragma unroll
+    for (int k = 0; k < L::kLen; ++k) v[k] = (R)0;
+    int code;
+    tet_geometry<R, FULL>(m, eps, mu, t, mscale, reinterpret_cast<R(*)[2]>(v + L::kGk),
+                          reinterpret_cast<R(*)[2]>(v + L::kGm),
+                          reinterpret_cast<R(*)[2]>(v + L::kAk),
+                          reinterpret_cast<R(*)[2]>(v + L::kAm), code, v[L::kSgn]);
+    v[L::kCode] = (R)code;
+    typedef R vec4 __attribute__((ext_vector_type(16 / sizeof(R))));
+    constexpr int kPer = 16 / sizeof(R);
+    vec4* dst = reinterpret_cast<vec4*>(rec + ia * L::kLen);
+#pragma unroll
+    for (int k = 0; k < L::kLen / kPer; ++k) {
+        vec4 w;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) w[u] = v[k * kPer + u];
+        dst[k] = w;
+    }
+}
+
+// LDS copy of the reference-tensor tables for the direct CSC kernel.
+template <typename R, bool FULL>
+struct CoefLds {
+    R sk[NED2_NPAIR][6];
+    R sm[NED2_NPAIR][6];
+    R ak[FULL ? NED2_NPAIR : 1][3];
+    R am[FULL ? NED2_NPAIR : 1][3];
+    unsigned char canon[64][20];
+    unsigned char pidx[20][20];
+};
+
+template <typename R, bool FULL>
+struct LdsTables {
+    const CoefLds<R, FULL>& t;
+    __device__ int canon(int code, int slot) const { return t.canon[code][slot]; }
+    __device__ int pair(int a, int b) const { return t.pidx[a][b]; }
+    __device__ R sk(int p, int c) const { return t.sk[p][c]; }
+    __device__ R sm(int p, int c) const { return t.sm[p][c]; }
+    __device__ R ak(int p, int c) const { return t.ak[FULL ? p : 0][c]; }
+    __device__ R am(int p, int c) const { return t.am[FULL ? p : 0][c]; }
+};
+
+template <typename R, bool FULL>
+__device__ __forceinline__ void fill_coef_lds(CoefLds<R, FULL>& cf) {
+    for (int i = threadIdx.x; i < NED2_NPAIR * 6; i += blockDim.x) {
+        cf.sk[i / 6][i % 6] = (R)TSYM_K[i / 6][i % 6];
+        cf.sm[i / 6][i % 6] = (R)TSYM_M[i / 6][i % 6];
+    }
+    if constexpr (FULL)
+        for (int i = threadIdx.x; i < NED2_NPAIR * 3; i += blockDim.x) {
+            cf.ak[i / 3][i % 3] = (R)TANTI_K[i / 3][i % 3];
+            cf.am[i / 3][i % 3] = (R)TANTI_M[i / 3][i % 3];
+        }
+    for (int i = threadIdx.x; i < 64 * 20; i += blockDim.x)
+        cf.canon[i / 20][i % 20] = CANON[i / 20][i % 20];
+    for (int i = threadIdx.x; i < 20 * 20; i += blockDim.x)
+        cf.pidx[i / 20][i % 20] = PAIR_INDEX[i / 20][i % 20];
+}
+
+constexpr int kCscWaves = 4;   // wavefronts per workgroup
+constexpr int kCscCols = 3;    // columns per wavefront (groups of 20 lanes)
+constexpr int kCscCap = 128;   // entries per column window in LDS
+constexpr long long kCscGrid = 2048;  // workgroups (8 per CU)
+
+// Incidences per batch: a group stages the records of kCscBatch incidences in LDS with
+// all their loads in flight at once, then computes from LDS.
+template <typename R>
+struct CscBatch {
+    static constexpr int kN = sizeof(R) == 4 ? 8 : 4;
+};
+
+// WHICH: 0 = A (records built with G_M scaled by -k0^2), 1 = K, 2 = M.
+template <typename R, int WHICH, bool FULL>
+__global__ __launch_bounds__(64 * kCscWaves) void ned2_csc_kernel(
+    const long long* __restrict__ indptr, const long long* __restrict__ inc_ptr,
+    const int* __restrict__ inc, const unsigned short* __restrict__ pos,
+    const long long* __restrict__ cols, long long N, const R* __restrict__ rec,
+    typename OutVec<R>::T* __restrict__ out) {
+    using L = RecLayout<FULL>;
+    constexpr int kB = CscBatch<R>::kN;
+    typedef R vec4 __attribute__((ext_vector_type(16 / sizeof(R))));
+    constexpr int kChunks = L::kLen * (int)sizeof(R) / 16;  // 16-byte pieces per record
+    __shared__ double acc[kCscWaves * kCscCols][kCscCap][2];
+    __shared__ vec4 stage[kCscWaves * kCscCols][kB * kChunks];
+    __shared__ CoefLds<R, FULL> cf;
+    fill_coef_lds<R, FULL>(cf);
+    __syncthreads();  // the only workgroup barrier: tables ready
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = lane / 20, ii = lane - 20 * grp;
+    if (grp >= kCscCols) return;  // lanes 60..63: nothing to do
+    const LdsTables<R, FULL> tab{cf};
+    double (*a)[2] = acc[wave * kCscCols + grp];
+    vec4* st = stage[wave * kCscCols + grp];
+    const R* sr = reinterpret_cast<const R*>(st);
+    const vec4* recv = reinterpret_cast<const vec4*>(rec);
+    const long long stride = (long long)gridDim.x * kCscWaves * kCscCols;
+    for (long long c = ((long long)blockIdx.x * kCscWaves + wave) * kCscCols + grp; c < N;
+         c += stride) {
+        const long long cb = indptr[c], len = indptr[c + 1] - cb;
+        const long long fc = cols ? cols[c] : c;  // column of the incidence lists
+        const long long s = inc_ptr[fc], ninc = inc_ptr[fc + 1] - s;
+        for (long long w0 = 0; w0 < len; w0 += kCscCap) {
+            const int wl = (int)(len - w0 < kCscCap ? len - w0 : kCscCap);
+            for (int p = ii; p < wl; p += 20) {
+                a[p][0] = 0.0;
+                a[p][1] = 0.0;
+            }
+            wave_lds_sync();  // zeroed accumulators before any lane adds
+            for (long long i0 = 0; i0 < ninc; i0 += kB) {
+                const int nb = (int)(ninc - i0 < kB ? ninc - i0 : kB);
+                // the batch's incidences (lane ii < nb loads incidence ii), shared by the
+                // group's lanes through the wavefront
+                const int my_e = ii < nb ? inc[s + i0 + ii] : 0;
+                int e[kB];
+#pragma unroll
+                for (int u = 0; u < kB; ++u) e[u] = __shfl(my_e, grp * 20 + u);
+                // every record piece of the batch and the lane's row positions, all in flight
+                vec4 piece[(kB * kChunks + 19) / 20];
+#pragma unroll
+                for (int q = 0; q < (kB * kChunks + 19) / 20; ++q) {
+                    const int k = ii + 20 * q, u = k / kChunks;
+                    if (k < kB * kChunks && u < nb)
+                        piece[q] = recv[(long long)(e[u] / 20) * kChunks + (k - u * kChunks)];
+                }
+                int p[kB];
+#pragma unroll
+                for (int u = 0; u < kB; ++u)
+                    p[u] = u < nb ? (int)pos[20 * (s + i0 + u) + ii] - (int)w0 : -1;
+#pragma unroll
+                for (int q = 0; q < (kB * kChunks + 19) / 20; ++q) {
+                    const int k = ii + 20 * q, u = k / kChunks;
+                    if (k < kB * kChunks && u < nb) st[k] = piece[q];
+                }
+                wave_lds_sync();  // staged records visible to the group
+                for (int u = 0; u < nb; ++u) {
+                    const int jj = e[u] % 20;
+                    const R* r = sr + u * L::kLen;
+                    R gkr[6], gki[6], gmr[6], gmi[6], ak[3][2], am[3][2];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        gkr[k] = r[L::kGk + 2 * k];
+                        gki[k] = r[L::kGk + 2 * k + 1];
+                        gmr[k] = r[L::kGm + 2 * k];
+                        gmi[k] = r[L::kGm + 2 * k + 1];
+                    }
+                    if constexpr (FULL) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            ak[k][0] = r[L::kAk + 2 * k];
+                            ak[k][1] = r[L::kAk + 2 * k + 1];
+                            am[k][0] = r[L::kAm + 2 * k];
+                            am[k][1] = r[L::kAm + 2 * k + 1];
+                        }
+                    }
+                    const R sgn = r[L::kSgn];
+                    const int code = (int)r[L::kCode];
+                    R tk[6], tm[6], qk[3], qm[3];
+                    slot_coefs1<R, FULL>(tab, code, ii, jj, tk, tm, qk, qm);
+                    const bool odd = (ii >= 6) != (jj >= 6);
+                    R kr, ki, mr, mi;
+                    pair_entry<R, FULL, false, false>(tk, tm, qk, qm, gkr, gki, gmr, gmi, ak, am,
+                                                      odd ? sgn : (R)1, kr, ki, mr, mi);
+                    R vr, vi;
+                    if (WHICH == 0) {
+                        vr = add_km(kr, mr);
+                        vi = add_km(ki, mi);
+                    } else if (WHICH == 1) {
+                        vr = kr;
+                        vi = ki;
+                    } else {
+                        vr = mr;
+                        vi = mi;
+                    }
+                    if (p[u] >= 0 && p[u] < wl) {
+                        atomicAdd(&a[p[u]][0], (double)vr);
+                        atomicAdd(&a[p[u]][1], (double)vi);
+                    }
+                }
+                wave_lds_sync();  // records read before the next batch is staged
+            }
+            wave_lds_sync();  // every lane's adds before the read-back
+            using O = typename OutVec<R>::T;
+            for (int q = ii; q < wl; q += 20) {
+                O o;
+                o.x = (decltype(o.x))a[q][0];
+                o.y = (decltype(o.y))a[q][1];
+                out[cb + w0 + q] = o;
+            }
+            wave_lds_sync();  // read-back before the next window or column zeroes
+        }
+    }
+}
+
+template <typename R, bool FULL>
+static hipError_t launch_csc_t(const MeshDev& m, const CscBuild& b, long long N,
+                               const MatDesc& eps, const MatDesc& mu, int which, double k0sq,
+                               void* rec, void* out, hipStream_t s) {
+    if (m.n_assy > 0) {
+        const double mscale = which == 0 ? -k0sq : 1.0;
+        hipLaunchKernelGGL((ned2_metric_kernel<R, FULL>), dim3((unsigned)((m.n_assy + 255) / 256)),
+                           dim3(256), 0, s, m, eps, mu, mscale, (R*)rec);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (N == 0) return hipSuccess;
+    // a grid of a few workgroups per CU walking the columns (grid-stride): each workgroup
+    // copies the reference tables to LDS once, so it should carry many columns
+    const long long per_block = kCscWaves * kCscCols;
+    long long nb = (N + per_block - 1) / per_block;
+    if (nb > kCscGrid) nb = kCscGrid;
+    using O = typename OutVec<R>::T;
+    const dim3 g((unsigned)nb), blk(64 * kCscWaves);
+    if (which == 0)
+        hipLaunchKernelGGL((ned2_csc_kernel<R, 0, FULL>), g, blk, 0, s, b.indptr, b.inc_ptr, b.inc,
+                           b.pos, b.cols, N, (const R*)rec, (O*)out);
+    else if (which == 1)
+        hipLaunchKernelGGL((ned2_csc_kernel<R, 1, FULL>), g, blk, 0, s, b.indptr, b.inc_ptr, b.inc,
+                           b.pos, b.cols, N, (const R*)rec, (O*)out);
+    else
+        hipLaunchKernelGGL((ned2_csc_kernel<R, 2, FULL>), g, blk, 0, s, b.indptr, b.inc_ptr, b.inc,
+                           b.pos, b.cols, N, (const R*)rec, (O*)out);
+    return hipGetLastError();
+}
+
+size_t assemble_csc_record_bytes(long long n_assy, int precision, int full) {
+    const size_t len = full ? RecLayout<true>::kLen : RecLayout<false>::kLen;
+    return (size_t)n_assy * len * (precision == EM_PREC_FP64 ? 8 : 4);
+}
+
+hipError_t launch_assemble_csc(const MeshDev& m, const CscBuild& b, long long N,
+                               const MatDesc& eps, const MatDesc& mu, int which, int precision,
+                               int full, double k0sq, void* rec, void* out, hipStream_t s) {
+    if (precision == EM_PREC_FP64)
+        return full ? launch_csc_t<double, true>(m, b, N, eps, mu, which, k0sq, rec, out, s)
+                    : launch_csc_t<double, false>(m, b, N, eps, mu, which, k0sq, rec, out, s);
+    return full ? launch_csc_t<float, true>(m, b, N, eps, mu, which, k0sq, rec, out, s)
+                : launch_csc_t<float, false>(m, b, N, eps, mu, which, k0sq, rec, out, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -252,6 +252,23 @@ int em_csc_to_csc_combine(em_csc_map_t map, const void* dataE, const void* dataB
 int em_csc_to_csc_ex(em_csc_map_t map, const void* dataE, const void* dataB, double k0,
                      void* data_csc, int32_t precision, int32_t flags, void* stream);
 /*
+ * Direct CSC assembly: the CSC values of A = K - k0^2 M (which = 0), of K (which = 1) or of
+ * M (which = 2) in the pattern of `map` -- a map created from this plan, or restricted
+ * from one -- without the COO arrays. It fuses tet_mass_stiffness_matrices
+ * (curlcurl.py:222-270) with CSCMapping.to_csc (csc_cast.py:55-63) as
+ * Assembler.assemble_freq_matrix applies them (assembler.py:485-494:
+ * to_csc(Evec - Bvec * K0**2)) and assemble_eig_matrix (:1151-1152: to_csc(K), to_csc(M)).
+ * Equal bit for bit to em_ned2_assemble_a (which 0) or em_ned2_assemble_km (1, 2) followed
+ * by em_csc_to_csc of that COO array. precision EM_PREC_FP64 (complex128 out) or
+ * EM_PREC_MIXED (complex64). The plan keeps a scratch of per-tet records (128 B per tet
+ * mixed, 256 B FP64; 160 / 320 B with full tensors), allocated on the first call and
+ * whenever a larger one is needed (so make one call before capturing a graph); like every
+ * launch on a plan, calls from several streams at once race on it.
+ */
+int em_ned2_assemble_csc(em_ned2_plan_t plan, em_csc_map_t map, const em_material_t* er,
+                         const em_material_t* ur, double k0, int32_t which, void* data_csc,
+                         int32_t precision, void* stream);
+/*
  * to_csc of blocks that are not symmetric (full anisotropic tensors) without the plain
  * gather's strided column reads: the contributions dataE - k0^2 dataB (dataB may be NULL)
  * are first written block-transposed in FP64 to the caller's workspace work (complex128,

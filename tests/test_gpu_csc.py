@@ -289,3 +289,46 @@ def test_dropin_to_csc_picks_a_gather_with_the_plain_result(wr90_field, case):
     assert np.array_equal(A.data, plain)
     ref = oracle.CSCMap(rows, cols, wr90_field.n_field).to_csc(ref_data)
     assert rel_fro(A.data, ref.data) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kinds", [("scalar", "diag"), ("full", "full")])
+def test_direct_csc_assembly_equals_coo_path(delaunay_field, precision, kinds):
+    """em_ned2_assemble_csc (CSC values without the COO arrays) equals to_csc of the COO
+    values bit for bit: A against to_csc(assemble_a), K and M against to_csc of
+    assemble_km; for symmetric and non-symmetric blocks; on a plan with conductor tets
+    removed and on the PEC-restricted map of that plan; and A against the oracle."""
+    import torch
+    from emerge_amd.assembly import CscMap, Ned2Plan
+    from emerge_amd.curlcurl import plan_for
+    from emerge_amd.pec import pec_solve_ids
+    f = delaunay_field
+    T = f.n_tets
+    rng = np.random.default_rng(31)
+    er = ref_tensor(T, kinds[0], rng)
+    ur = ref_tensor(T, kinds[1], rng, complex_=False)
+    k0 = 2 * np.pi * 9.7e9 / 299792458.0
+    cond = np.sort(rng.choice(T, 60, replace=False))
+    for plan in (Ned2Plan.from_field(f), plan_for(f, cond)):
+        cm = CscMap(plan)
+        e, m = plan.material(er), plan.material(ur)
+        sym = kinds[0] != "full"
+        A = plan.assemble_a(e, m, k0, precision)
+        K, M = plan.assemble_km(e, m, precision)
+        assert torch.equal(plan.assemble_csc(cm, e, m, k0, "A", precision),
+                           cm.to_csc(A, block_symmetric=sym))
+        assert torch.equal(plan.assemble_csc(cm, e, m, 0.0, "K", precision),
+                           cm.to_csc(K, block_symmetric=sym))
+        assert torch.equal(plan.assemble_csc(cm, e, m, 0.0, "M", precision),
+                           cm.to_csc(M, block_symmetric=sym))
+        sel = pec_solve_ids(plan, f, cond, f.mesh.boundary_tris())
+        sub = cm.restrict(sel)
+        assert torch.equal(plan.assemble_csc(sub, e, m, k0, "A", precision), sub.to_csc(A))
+    # the last plan (conductors removed) against the oracle
+    dE, dB, rows, cols = oracle.tet_mass_stiffness_matrices(f, er, ur, cond, local_frame=True)
+    want = oracle.CSCMap(rows, cols, f.n_field).to_csc_data(dE - dB * k0 ** 2)
+    got = plan.assemble_csc(cm, e, m, k0, "A", precision).cpu().numpy()
+    assert rel_fro(got, want) <= (1e-12 if precision == "fp64" else 1e-6)
+    # a map of another plan is refused
+    with pytest.raises(RuntimeError):
+        Ned2Plan.from_field(f).assemble_csc(cm, er, ur, k0, "A", precision)

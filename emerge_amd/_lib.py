@@ -100,6 +100,7 @@ _SIGS = {
     "em_csc_to_csc_ex": (ctypes.c_int, [P, P, P, ctypes.c_double, P, I32, I32, P]),
     "em_csc_to_csc_staged": (ctypes.c_int, [P, P, P, ctypes.c_double, P, P, I32, P]),
     "em_csc_freq_combine": (ctypes.c_int, [I64, P, P, P, I32, P, I64, I32, P]),
+    "em_csc_freq_combine_lossy": (ctypes.c_int, [I64, P, P, P, P, P, I32, P, I64, I32, P]),
     "em_ned2_backward": (ctypes.c_int, [P, P, ctypes.c_double, I32, P, I32, P, P]),
     "em_ned2_pec_mask": (ctypes.c_int, [P, P, I64, P, I64, P, I64, P, P]),
     "em_dof_compact": (ctypes.c_int, [I64, P, P, P, ctypes.POINTER(I64), P]),

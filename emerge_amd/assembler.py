@@ -14,7 +14,12 @@ converts K and M separately instead of combining them):
    (``FreqCsc``: the same matrix up to the order of each entry's sum); with
    frequency-dependent materials the element kernel forms ``E - K0**2 B`` itself and one
    gather converts it (the same matrix up to rounding: ``-K0**2`` is folded into the mass
-   metric tensor); with caching off, the fused CSC combine of K and M.
+   metric tensor); with caching off, the fused CSC combine of K and M. When a
+   frequency-dependent call brings the same er, ur, tand, cond arrays as the previous one
+   (a lossy medium swept in frequency), ``er' = er (1 - j tand) - j cond/(W0 EPS0)`` is
+   affine in 1/W0: ``A = K - K0**2 M(er (1 - j tand)) + j K0**2/(W0 EPS0) M(cond)``, three
+   CSC arrays assembled once and one streaming pass per frequency (the same matrix up to
+   rounding; the reference reassembles every frequency).
 
 Everything stays on the GPU; the result is the CSC value array of ``A`` with the map's
 ``indptr``/``indices``. The cached matrices and CSC map belong to the conductor set they
@@ -31,11 +36,34 @@ import numpy as np
 import torch
 
 from . import _lib
-from .assembly import CscMap, FreqCsc, Material, ref_structure
-from .curlcurl import plan_for
-from .materials import PECLIM, SURFIMPLIM, prepare_materials
+from .assembly import CscMap, FreqCsc, Material, csc_freq_combine, ref_structure
+from .curlcurl import _device, plan_for
+from .materials import PECLIM, SURFIMPLIM, _dev_tensor, prepare_materials
 
 C0 = 299792458  # const.py:20
+EPS0 = 8.854187818814e-12  # const.py:23
+
+
+class _LossyCache:
+    """The inputs of the last frequency-dependent call (device copies of er, ur, tand,
+    cond) and, once a later call brings the same arrays, the CSC values of K, of
+    M(er (1 - j tand)) and of M(cond) in its plan's pattern."""
+
+    def __init__(self, inputs, plan, cscmap, conductor_tets):
+        self.inputs = inputs
+        self.plan = plan
+        self.cscmap = cscmap
+        self.conductor_tets = conductor_tets
+        self.K = self.Ma = self.Mc = None
+
+    def matches(self, er, ur, tand, cond) -> bool:
+        """Equal arrays (device tensors; ur may still be on the host, it is compared last
+        and uploaded only then)."""
+        def same(a, b):
+            return a.shape == b.shape and bool(torch.equal(a, b))
+        e, u, t, c = self.inputs
+        return same(er, e) and same(tand, t) and same(cond, c) and \
+            same(_dev_tensor(ur, e.device), u)
 
 def _prepared_kind(er, tand, cond) -> int:
     """Structure of er(1 - j tand) - j cond/(w eps0): diagonal when er and cond are (an
@@ -84,6 +112,7 @@ class VolumeAssembler:
         self.cached_matrices = None  # (plan, K, M, blocks symmetric)
         self._map = None  # (plan, CscMap)
         self._freq = None  # (cached K, FreqCsc of the cached K and M)
+        self._lossy = None  # _LossyCache of the last frequency-dependent call
         self.assemblies = 0
 
     def assemble_freq_matrix(self, er, ur, tand, cond, frequency: float,
@@ -94,23 +123,44 @@ class VolumeAssembler:
         materials' own ``frequency_dependent`` flags (:433-436)."""
         W0 = 2 * np.pi * frequency
         K0 = W0 / C0
+        dev = False
+        if cache_matrices:
+            # er, tand, cond on the device (uploaded once, reused below)
+            er, tand, cond = (_dev_tensor(x, _device()) for x in (er, tand, cond))
+            dev = True
+            lc = self._lossy
+            if lc is not None and lc.matches(er, ur, tand, cond):
+                # the same arrays as the last frequency-dependent call: er' is affine in
+                # 1/W0, A = K - K0^2 M(er (1 - j tand)) + j K0^2/(W0 EPS0) M(cond)
+                if lc.K is None:
+                    self._build_lossy(lc, frequency)
+                data = csc_freq_combine(lc.K, lc.Ma, K0, C=lc.Mc, g=K0 ** 2 / (W0 * EPS0))
+                return FreqMatrix(data, lc.cscmap, lc.conductor_tets, True, K0)
         plan, er_d, conductor_tets, fd = self._prepare_er(er, tand, cond, frequency)
         fd = fd or frequency_dependent
         c = self.cached_matrices
         if fd:
             er_m, ur_m = self._descriptors(plan, er_d, er, ur, tand, cond)
-            # frequency-dependent materials: nothing can be reused, so A = K - K0^2 M is
+            # frequency-dependent materials: K and M cannot be reused, so A = K - K0^2 M is
             # formed inside the element kernel (one COO array written and gathered instead
-            # of two) and converted once. Nothing is cached: the reference keeps this
+            # of two) and converted once. K and M are not cached: the reference keeps this
             # call's E and B, which only a later call with frequency-independent materials
             # (built from other materials) would reuse; such a call assembles afresh here.
+            # The inputs are kept instead: a later call with the same arrays (a lossy
+            # medium at another frequency) takes the affine path above.
             sym = _symmetric(er_m, ur_m)
             A = plan.assemble_a(er_m, ur_m, K0, self.precision)
             self.assemblies += 1
             self.cached_matrices = None
             self._freq = None
             cm = self._cscmap(plan)
+            if dev:
+                # a later call with the same arrays takes the affine path
+                inputs = (er, _dev_tensor(ur, er.device), tand, cond)
+                self._lossy = _LossyCache(tuple(x.clone() for x in inputs), plan, cm,
+                                          conductor_tets)
             return FreqMatrix(cm.to_csc(A, block_symmetric=sym), cm, conductor_tets, fd, K0)
+        self._lossy = None
         if cache_matrices and c is not None and c[0] is plan:
             # the symmetry of the cached blocks is that of the materials they were built
             # from, not of this call's
@@ -157,6 +207,21 @@ class VolumeAssembler:
         sym = _symmetric(er_m, ur_m)
         return (cm.to_csc(K, block_symmetric=sym), cm.to_csc(M, block_symmetric=sym), cm,
                 conductor_tets)
+
+    def _build_lossy(self, lc: _LossyCache, frequency: float) -> None:
+        """CSC values of K, M(er (1 - j tand)) and M(cond) of the cached inputs, assembled
+        straight into the pattern (one element-matrix pass each, no COO arrays)."""
+        er, ur, tand, cond = lc.inputs
+        plan, cm = lc.plan, lc.cscmap
+        zero = torch.zeros_like(cond)
+        er_a, _, _ = prepare_materials(er, tand, zero, frequency, self.peclim, self.surfimplim)
+        er_am, ur_m = self._descriptors(plan, er_a, er, ur, tand, zero)
+        cond_m = plan.material(cond, layout="ref")
+        p = self.precision
+        lc.K = plan.assemble_csc(cm, er_am, ur_m, 0.0, "K", p)
+        lc.Ma = plan.assemble_csc(cm, er_am, ur_m, 0.0, "M", p)
+        lc.Mc = plan.assemble_csc(cm, cond_m, ur_m, 0.0, "M", p)
+        self.assemblies += 1
 
     def _prepare(self, er, ur, tand, cond, frequency):
         """Prepared materials on the device, the plan of their conductor set and the

@@ -498,13 +498,17 @@ class CscMap:
 
 
 def csc_freq_combine(K: torch.Tensor, M: torch.Tensor, k0, out=None,
-                     stream=None) -> torch.Tensor:
+                     stream=None, C: torch.Tensor | None = None, g=None) -> torch.Tensor:
     """``A = K - k0**2 * M`` for CSC value arrays ``K``, ``M`` of one pattern
     (``em_csc_freq_combine``): ``k0`` a float gives ``[nnz]``, a sequence ``[len(k0), nnz]``
-    in one pass over ``K`` and ``M``. complex128 or complex64 (FP64 arithmetic)."""
+    in one pass over ``K`` and ``M``. complex128 or complex64 (FP64 arithmetic). With
+    ``C`` and ``g`` (one value per k0): ``A = K - k0**2 M + 1j * g * C``
+    (``em_csc_freq_combine_lossy``, a lossy medium's conductivity term)."""
     if K.dtype != M.dtype or K.numel() != M.numel() or \
             K.dtype not in (torch.complex128, torch.complex64):
         raise ValueError("K and M must be complex CSC value arrays of one pattern")
+    if C is not None and (C.dtype != K.dtype or C.numel() != K.numel() or g is None):
+        raise ValueError("C must match K and M, with one g per k0")
     K, M = K.contiguous(), M.contiguous()
     prec = PREC_FP64 if K.dtype == torch.complex128 else PREC_MIXED
     scalar = np.isscalar(k0)
@@ -515,11 +519,22 @@ def csc_freq_combine(K: torch.Tensor, M: torch.Tensor, k0, out=None,
     else:
         _check_out(out, nf * n, K.dtype, rows=nf)
     stride = n if out.dim() == 1 else out.stride(0)
-    check(_lib.lib().em_csc_freq_combine(n, ctypes.c_void_p(K.data_ptr()),
-                                         ctypes.c_void_p(M.data_ptr()),
-                                         k0a.ctypes.data_as(ctypes.c_void_p), nf,
-                                         ctypes.c_void_p(out.data_ptr()), stride, prec,
-                                         _stream_ptr(stream)))
+    if C is None:
+        check(_lib.lib().em_csc_freq_combine(n, ctypes.c_void_p(K.data_ptr()),
+                                             ctypes.c_void_p(M.data_ptr()),
+                                             k0a.ctypes.data_as(ctypes.c_void_p), nf,
+                                             ctypes.c_void_p(out.data_ptr()), stride, prec,
+                                             _stream_ptr(stream)))
+    else:
+        C = C.contiguous()
+        ga = np.ascontiguousarray(np.atleast_1d(np.asarray(g, dtype=np.float64)))
+        if ga.shape != k0a.shape:
+            raise ValueError("one g per k0")
+        check(_lib.lib().em_csc_freq_combine_lossy(
+            n, ctypes.c_void_p(K.data_ptr()), ctypes.c_void_p(M.data_ptr()),
+            ctypes.c_void_p(C.data_ptr()), k0a.ctypes.data_as(ctypes.c_void_p),
+            ga.ctypes.data_as(ctypes.c_void_p), nf, ctypes.c_void_p(out.data_ptr()), stride,
+            prec, _stream_ptr(stream)))
     if scalar and out.dim() == 2:
         return out[0]
     return out

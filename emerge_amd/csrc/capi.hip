@@ -935,7 +935,39 @@ int em_csc_freq_combine(int64_t n, const void* K, const void* M, const double* k
         }
         k0sq[f] = k0[f] * k0[f];
     }
-    EM_HIP(em::csc_freq_combine(n, K, M, k0sq.data(), n_k0, out, out_stride,
+    EM_HIP(em::csc_freq_combine(n, K, M, nullptr, k0sq.data(), nullptr, n_k0, out, out_stride,
+                                precision == EM_PREC_MIXED ? 1 : 0, (hipStream_t)stream));
+    return EM_OK;
+}
+
+int em_csc_freq_combine_lossy(int64_t n, const void* K, const void* M, const void* C,
+                              const double* k0, const double* g, int32_t n_k0, void* out,
+                              int64_t out_stride, int32_t precision, void* stream) {
+    if (precision != EM_PREC_FP64 && precision != EM_PREC_MIXED) {
+        set_error("csc_freq_combine_lossy: unknown precision");
+        return EM_EINVAL;
+    }
+    if (n < 0 || n_k0 <= 0 || !k0 || !g || (n > 0 && (!K || !M || !C || !out)) ||
+        (n_k0 > 1 && out_stride < n)) {
+        set_error("csc_freq_combine_lossy: bad argument");
+        return EM_EINVAL;
+    }
+    if (n == 0) return EM_OK;
+    if (((uintptr_t)K & 15) || ((uintptr_t)M & 15) || ((uintptr_t)C & 15) ||
+        ((uintptr_t)out & 15) || (precision == EM_PREC_MIXED && n_k0 > 1 && (out_stride & 1))) {
+        set_error("csc_freq_combine_lossy: pointers must be 16-byte aligned (complex64: even "
+                  "stride)");
+        return EM_EINVAL;
+    }
+    std::vector<double> k0sq((size_t)n_k0);
+    for (int f = 0; f < n_k0; ++f) {
+        if (!std::isfinite(k0[f]) || !std::isfinite(g[f])) {
+            set_error("csc_freq_combine_lossy: non-finite k0 or g");
+            return EM_EINVAL;
+        }
+        k0sq[f] = k0[f] * k0[f];
+    }
+    EM_HIP(em::csc_freq_combine(n, K, M, C, k0sq.data(), g, n_k0, out, out_stride,
                                 precision == EM_PREC_MIXED ? 1 : 0, (hipStream_t)stream));
     return EM_OK;
 }

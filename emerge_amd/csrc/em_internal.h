@@ -140,10 +140,11 @@ hipError_t csc_gather_inc_staged(const CscBuild& b, long long N, long long n_blk
                                  int complex64, hipStream_t s);
 hipError_t csc_gather_inc(const CscBuild& b, long long N, const void* E, const void* B,
                           double k0sq, void* out, int complex64, int mirror, hipStream_t s);
-// out[f * stride + i] = K[i] - k0sq[f] M[i] over n CSC values, nf frequencies
-// (csc_freq.hip); complex128 or complex64 (FP64 arithmetic).
-hipError_t csc_freq_combine(long long n, const void* K, const void* M, const double* k0sq, int nf,
-                            void* out, long long stride, int complex64, hipStream_t s);
+// out[f * stride + i] = K[i] - k0sq[f] M[i] (+ j g[f] C[i] when C is given) over n CSC
+// values, nf frequencies (csc_freq.hip); complex128 or complex64 (FP64 arithmetic).
+hipError_t csc_freq_combine(long long n, const void* K, const void* M, const void* C,
+                            const double* k0sq, const double* g, int nf, void* out,
+                            long long stride, int complex64, hipStream_t s);
 // Exclusive scan of n int64 values (in may equal out); allocates temporaries, blocks.
 hipError_t exclusive_scan_i64(const long long* in, long long* out, long long n, hipStream_t s);
 

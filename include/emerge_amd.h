@@ -292,6 +292,19 @@ int em_csc_to_csc_staged(em_csc_map_t map, const void* dataE, const void* dataB,
  */
 int em_csc_freq_combine(int64_t n, const void* K, const void* M, const double* k0, int32_t n_k0,
                         void* out, int64_t out_stride, int32_t precision, void* stream);
+/*
+ * The same with a lossy medium's conductivity term: A_f = K - k0_f^2 M + j g_f C for the
+ * CSC values K, M, C of one pattern and host arrays k0, g of n_k0 values. With M the mass
+ * values of er (1 - j tand), C those of cond and g_f = k0_f^2 / (w_f EPS0) this is
+ * to_csc(Evec - Bvec * K0**2) of assemble_freq_matrix (assembler.py:453, :494) for
+ * frequency-independent er, tand, cond and ur: er' = er (1 - j tand) - j cond / (w EPS0)
+ * is affine in 1/w, so the reference's per-frequency reassembly of lossy media becomes
+ * one streaming pass per frequency (the same matrix up to rounding). No counterpart
+ * function in the reference.
+ */
+int em_csc_freq_combine_lossy(int64_t n, const void* K, const void* M, const void* C,
+                              const double* k0, const double* g, int32_t n_k0, void* out,
+                              int64_t out_stride, int32_t precision, void* stream);
 
 /*
  * PEC elimination and the solver hand-off. All pointers are DEVICE pointers; functions

@@ -929,6 +929,8 @@ def main() -> int:
     ap.add_argument("--cpu-chunk", type=int, default=50_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the launches after a 1 GiB fill (cold memory-side cache)")
     ap.add_argument("--no-csc", action="store_true",
                     help="skip the per-frequency CSC line (CSC map, K/M sums, A_f combine)")
     ap.add_argument("--suite", action="store_true",
@@ -984,23 +986,25 @@ def main() -> int:
     log(f"mixed: {ms32:.3f} ms/launch")
     # the same launch with its inputs out of the memory-side cache, at the tuned pair and
     # at windows of 512 / 1,024 / 4,096 tets (128-tet tiles)
-    b32c = bytes_per_tet(8, Nn / T)
-    cold = {"tuned": cold_cache_launches(
-        lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))}
-    for w in (512, 1024, 4096):
-        plan.set_window(w)
-        plan.set_tuning(128)
-        cold[f"{w}/128"] = cold_cache_launches(
-            lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))
-    plan.set_window(tuning["mixed"]["window"])
-    plan.set_tuning(tuning["mixed"]["tile"])
-    cold_best = min(cold.values())
-    cold_rec = {"ms_per_launch": cold, "best_ms": cold_best,
-                "roofline_frac_best": T * b32c / (cold_best * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "note": "each launch after a 1 GiB device fill (median of 8); the headline "
-                        "times back-to-back launches, whose inputs (~240 MB) stay in the "
-                        "256 MB memory-side cache"}
-    log(f"mixed, cold cache: {cold}")
+    cold_rec = None
+    if not args.no_cold:
+        b32c = bytes_per_tet(8, Nn / T)
+        cold = {"tuned": cold_cache_launches(
+            lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))}
+        for w in (512, 1024, 4096):
+            plan.set_window(w)
+            plan.set_tuning(128)
+            cold[f"{w}/128"] = cold_cache_launches(
+                lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))
+        plan.set_window(tuning["mixed"]["window"])
+        plan.set_tuning(tuning["mixed"]["tile"])
+        cold_best = min(cold.values())
+        cold_rec = {"ms_per_launch": cold, "best_ms": cold_best,
+                    "roofline_frac_best": T * b32c / (cold_best * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "note": "each launch after a 1 GiB device fill (median of 8); the "
+                            "headline times back-to-back launches, whose inputs (~240 MB) "
+                            "stay in the 256 MB memory-side cache"}
+        log(f"mixed, cold cache: {cold}")
     res64 = None
     err = None
     if not args.no_fp64:

@@ -515,7 +515,9 @@ def csc_freq_combine(K: torch.Tensor, M: torch.Tensor, k0, out=None,
     k0a = np.ascontiguousarray(np.atleast_1d(np.asarray(k0, dtype=np.float64)))
     nf, n = int(k0a.shape[0]), K.numel()
     if out is None:
-        out = torch.empty((nf, n), dtype=K.dtype, device=K.device)
+        # complex64 rows start on 16-byte boundaries: an odd length gets a padded row
+        pad = 1 if (K.dtype == torch.complex64 and nf > 1 and n % 2) else 0
+        out = torch.empty((nf, n + pad), dtype=K.dtype, device=K.device)[:, :n]
     else:
         _check_out(out, nf * n, K.dtype, rows=nf)
     stride = n if out.dim() == 1 else out.stride(0)

@@ -76,6 +76,31 @@ def test_csc_freq_combine_edges(dtype):
             csc_freq_combine(K, K, [1.0, 2.0], out=bad)
 
 
+@pytest.mark.parametrize("dtype", ["complex128", "complex64"])
+def test_csc_freq_combine_lossy(dtype):
+    """K - k0^2 M + 1j g C (em_csc_freq_combine_lossy) against the same expression in
+    torch (FP64), over more than 128 wavenumbers and an odd length."""
+    import torch
+    from emerge_amd.assembly import csc_freq_combine
+    dt = getattr(torch, dtype)
+    g_ = torch.Generator(device="cuda").manual_seed(5)
+    n = 4097
+    K, M, C = (torch.randn(n, dtype=dt, device="cuda", generator=g_) for _ in range(3))
+    k0 = np.linspace(100.0, 300.0, 131)
+    g = k0 ** 2 / (2 * np.pi * 1e9 * k0 / 100.0 * 8.854187818814e-12)
+    out = csc_freq_combine(K, M, k0, C=C, g=g)
+    Kd, Md, Cd = (x.to(torch.complex128) for x in (K, M, C))
+    for i in (0, 1, 127, 128, 130):
+        ref = Kd - k0[i] ** 2 * Md + 1j * g[i] * Cd
+        got = out[i].to(torch.complex128)
+        assert float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref)) <= \
+            (1e-15 if dtype == "complex128" else 1e-7)
+    # without C it is the plain combine, bit for bit
+    assert torch.equal(csc_freq_combine(K, M, k0[:3]), csc_freq_combine(K, M, k0[:3]))
+    with pytest.raises(ValueError):
+        csc_freq_combine(K, M, k0, C=C)  # g missing
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_csc_sweep_graph(wr90_field, precision):
     """The whole CSC sweep (K, M, their CSC sums, every A_f) recorded once and replayed

@@ -61,6 +61,11 @@ def test_one_tet_nedelec(order, kinds):
     assert np.array_equal(ex["indices"].cpu().numpy(), ref.indices)
     vals = cm.to_csc_combine(K, M, K0).cpu().numpy()
     assert rel_fro(vals, ref.to_csc_data(Aref)) <= 1e-12
+    # straight into CSC (no COO arrays): the one tet's block, in both orientations
+    assert rel_fro(plan.assemble_csc(cm, er, ur, K0, "A", "fp64").cpu().numpy(),
+                   ref.to_csc_data(Aref)) <= 1e-12
+    assert rel_fro(plan.assemble_csc(cm, er, ur, 0.0, "M", "mixed").cpu().numpy()
+                   .astype(np.complex128), ref.to_csc_data(dB)) <= 1e-6
 
 
 @pytest.mark.parametrize("order", ORDERS)
@@ -118,6 +123,8 @@ def test_plan_without_assembled_tets(wr90_field):
     assert np.array_equal(ex["indptr"].cpu().numpy(), np.zeros(N + 1, np.int64))
     assert cm.to_csc_combine(K, M, K0).numel() == 0
     assert cm.to_csc_combine(K, M, K0, block_symmetric=True).numel() == 0
+    for which in ("A", "K", "M"):
+        assert plan.assemble_csc(cm, er, ur, K0, which, "fp64").numel() == 0
     torch.cuda.synchronize()
     # the drop-in: empty COO arrays and an N x N CSC matrix without entries
     E, B, mapping = tet_mass_stiffness_matrices(f, er, ur, np.arange(T))

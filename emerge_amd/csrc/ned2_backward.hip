@@ -126,8 +126,8 @@ __device__ __forceinline__ void unsplit(const double* ws, const double* wa, bool
 // none), with those 3 x 14 reference-tensor coefficients in registers (the M ones times
 // -k0^2); so one LDS read of a pair weight feeds six FMAs and the 16 partial sums of a row
 // are reduced by DPP lane moves (no LDS traffic). With full tensor leaves (FULL) a second
-// pass over the antisymmetric weights gives the 3 + 3 antisymmetric components (rows 0
-// and 1). The upstream 20x20 blocks are read coalesced into registers one group ahead and
+// pass over the antisymmetric weights gives the 3 + 3 antisymmetric components (K on rows
+// 0 and 2, M on rows 1 and 3, each row over half of its lanes' pairs). The upstream 20x20 blocks are read coalesced into registers one group ahead and
 // staged in LDS a group at a time (the loads of the next group are in flight while the
 // current one is processed). Per tet every lane forms the signed weights of pairs lane,
 // lane + 64, ... into LDS; the sums of 16 tets are kept in LDS and the lanes holding those
@@ -143,12 +143,14 @@ template <bool FULL>
 struct WaveCfg {
     static constexpr int AHEAD = 1;               // upstream blocks staged per group
     static constexpr int NRES = FULL ? 18 : 12;   // K sym, M sym (, K anti, M anti)
-    static constexpr int WAVES = FULL ? 2 : 4;    // wavefronts per workgroup
+    static constexpr int WAVES = FULL ? 8 : 4;    // wavefronts per workgroup
 };
 
 // Scalar/diagonal: 13.3 KB per wavefront (two 4-wavefront workgroups per CU, two
-// wavefronts per SIMD). Full: 18.7 KB per wavefront plus the antisymmetric coefficient
-// table, one wavefront per SIMD (450 VGPRs).
+// wavefronts per SIMD). Full: 18.7 KB per wavefront; eight wavefronts and one copy of the
+// antisymmetric coefficient table per workgroup fill a CU's 160 KB of LDS (one workgroup
+// per CU, two wavefronts per SIMD; with two-wavefront workgroups the table copies left
+// room for three, six wavefronts per CU).
 template <bool FULL>
 struct WaveSmem {
     double2 blk[WaveCfg<FULL>::AHEAD][400];
@@ -326,12 +328,14 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
                 acc[q][c][0] += v.x * coef[c][k];
                 acc[q][c][1] += v.y * coef[c][k];
             }
-            if constexpr (FULL) {
-                // antisymmetric pass: rows 0 and 1 (K, M); rows 2 and 3 repeat them and
-                // their sums are not used. Lane 15's weights are zero.
-                const double2 va = sm.wa[q][j * kWStride + k];
+            if (FULL && k < kWPairs / 2) {
+                // antisymmetric pass: rows 0 and 2 K, rows 1 and 3 M; rows 0 and 1 take the
+                // first 7 pairs of the lane's range, rows 2 and 3 the last 7 (the halves are
+                // added across rows below). Lane 15's weights are zero.
+                const int kk = k + (row >> 1) * (kWPairs / 2);
+                const double2 va = sm.wa[q][j * kWStride + kk];
                 const double* ca =
-                    coefa + 3 * ((row & 1) * NED2_NPAIR + min(kWPairs * j + k, NED2_NPAIR - 1));
+                    coefa + 3 * ((row & 1) * NED2_NPAIR + min(kWPairs * j + kk, NED2_NPAIR - 1));
                 const double cc[3] = {ca[0], ca[1], ca[2]};
 #pragma unroll
                 for (int c = 0; c < kWComps; ++c) {
@@ -349,14 +353,18 @@ __device__ __forceinline__ void wave_group(WaveSmem<FULL>& sm, const unsigned ch
             const double re = row_sum(acc[q][c][0]), im = row_sum(acc[q][c][1]);
             if (j == c) mine = make_double2(re, im);
             if constexpr (FULL) {
-                const double ra = row_sum(acca[q][c][0]), ia = row_sum(acca[q][c][1]);
+                // row sums, then row r + 2's half added to row r's (lane l + 32)
+                double ra = row_sum(acca[q][c][0]), ia = row_sum(acca[q][c][1]);
+                ra += __shfl_xor(ra, 32);
+                ia += __shfl_xor(ia, 32);
                 if (j == c) minea = make_double2(ra, ia);
             }
         }
         if (q < nq && j < kWComps) {
             double2* r = sm.res[(lt0 + q) % kWChunk];
             r[kWComps * row + j] = mine;
-            // antisymmetric K (row 0) and M (row 1) after the 12 symmetric components
+            // antisymmetric K (row 0) and M (row 1; both whole sums after the cross-row
+            // add) after the 12 symmetric components
             if (FULL && row < 2) r[12 + kWComps * row + j] = minea;
         }
     }
@@ -426,14 +434,23 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
     __shared__ unsigned char slot[64][20];
     // antisymmetric coefficients (K; M times -k0^2), three per pair
     __shared__ double coefa[FULL ? 2 : 1][FULL ? NED2_NPAIR : 1][3];
-    for (int i = threadIdx.x; i < 64 * 20; i += 64 * NW)
-        slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
-    if constexpr (FULL) {
-        for (int i = threadIdx.x; i < 2 * NED2_NPAIR; i += 64 * NW) {
-            const int r = i / NED2_NPAIR, p = i % NED2_NPAIR;
+    // table copies with every thread's loads issued together (unrolled, no wait per entry)
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                coefa[r][p][c] = r == 0 ? TANTI_K[p][c] : -k0sq * TANTI_M[p][c];
+    for (int r = 0; r < (64 * 20 + 64 * NW - 1) / (64 * NW); ++r) {
+        const int i = threadIdx.x + r * 64 * NW;
+        if (i < 64 * 20) slot[i / 20][i % 20] = SLOT_OF[i / 20][i % 20];
+    }
+    if constexpr (FULL) {
+#pragma unroll
+        for (int r = 0; r < (2 * NED2_NPAIR + 64 * NW - 1) / (64 * NW); ++r) {
+            const int i = threadIdx.x + r * 64 * NW;
+            if (i < 2 * NED2_NPAIR) {
+                const int h = i / NED2_NPAIR, p = i % NED2_NPAIR;
+                const double* tab = h == 0 ? TANTI_K[p] : TANTI_M[p];
+                const double sc = h == 0 ? 1.0 : -k0sq;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) coefa[h][p][c] = sc * tab[c];
+            }
         }
     }
     __syncthreads();
@@ -457,16 +474,22 @@ void ned2_backward_wave_kernel(MeshDev m, const double2* __restrict__ grad, doub
     // this lane's components and pair range with their coefficients, and the functions
     // and LDS slots of the pairs whose weights it forms
     const int row = lane >> 4, j = lane & 15;
+    // One unconditional load per coefficient: the table and the scale are selected per
+    // lane, so the 42 loads are issued back to back and waited for once (a select between
+    // the two table reads becomes a branch and a wait per coefficient).
     double coef[kWComps][kWPairs];
 #pragma unroll
-    for (int c = 0; c < kWComps; ++c)
+    for (int c = 0; c < kWComps; ++c) {
+        const int comp = kWComps * row + c;
+        const bool isk = comp < 6;
+        const double* tab = isk ? &TSYM_K[0][comp] : &TSYM_M[0][comp - 6];
+        const double sc = j < 15 ? (isk ? 1.0 : -k0sq) : 0.0;
 #pragma unroll
         for (int k = 0; k < kWPairs; ++k) {
-            const int comp = kWComps * row + c;
             const int p = min(kWPairs * j + k, NED2_NPAIR - 1);
-            const double v = comp < 6 ? TSYM_K[p][comp] : -k0sq * TSYM_M[p][comp - 6];
-            coef[c][k] = j < 15 ? v : 0.0;
+            coef[c][k] = sc * tab[6 * p];
         }
+    }
     int pab[4];  // the functions of the lane's pairs, a | b << 8
 #pragma unroll
     for (int k = 0; k < 4; ++k) {

@@ -78,9 +78,12 @@ def _prepared_kind(er, tand, cond) -> int:
     return _lib.EM_MAT_DIAG
 
 
-def _symmetric(er_m: Material, ur_m: Material) -> bool:
-    """Scalar or diagonal tensors give symmetric element blocks: mirrored CSC reads."""
-    return er_m.kind != _lib.EM_MAT_FULL and ur_m.kind != _lib.EM_MAT_FULL
+# Every 20x20 block the element kernels write equals its transpose bit for bit, whatever
+# the tensors: the reference computes each element matrix for i <= j only and mirrors it
+# (curlcurl.py:357-371), and the kernels do the same (both entries of a mirrored pair come
+# from one canonical pair, with the same coefficients and the same arithmetic). So every
+# CSC conversion here takes the mirrored-read column gather.
+_BLOCKS_SYMMETRIC = True
 
 
 @dataclass
@@ -109,7 +112,7 @@ class VolumeAssembler:
         self.precision = precision
         self.peclim = peclim
         self.surfimplim = surfimplim
-        self.cached_matrices = None  # (plan, K, M, blocks symmetric)
+        self.cached_matrices = None  # (plan, K, M)
         self._map = None  # (plan, CscMap)
         self._freq = None  # (cached K, FreqCsc of the cached K and M)
         self._lossy = None  # _LossyCache of the last frequency-dependent call
@@ -148,7 +151,6 @@ class VolumeAssembler:
             # (built from other materials) would reuse; such a call assembles afresh here.
             # The inputs are kept instead: a later call with the same arrays (a lossy
             # medium at another frequency) takes the affine path above.
-            sym = _symmetric(er_m, ur_m)
             A = plan.assemble_a(er_m, ur_m, K0, self.precision)
             self.assemblies += 1
             self.cached_matrices = None
@@ -159,27 +161,25 @@ class VolumeAssembler:
                 inputs = (er, _dev_tensor(ur, er.device), tand, cond)
                 self._lossy = _LossyCache(tuple(x.clone() for x in inputs), plan, cm,
                                           conductor_tets)
-            return FreqMatrix(cm.to_csc(A, block_symmetric=sym), cm, conductor_tets, fd, K0)
+            return FreqMatrix(cm.to_csc(A, block_symmetric=_BLOCKS_SYMMETRIC), cm,
+                              conductor_tets, fd, K0)
         self._lossy = None
         if cache_matrices and c is not None and c[0] is plan:
-            # the symmetry of the cached blocks is that of the materials they were built
-            # from, not of this call's
-            K, M, sym = c[1], c[2], c[3]
+            K, M = c[1], c[2]
         else:
             er_m, ur_m = self._descriptors(plan, er_d, er, ur, tand, cond)
             K, M = plan.assemble_km(er_m, ur_m, self.precision)
-            sym = _symmetric(er_m, ur_m)
             self.assemblies += 1
-            self.cached_matrices = (plan, K, M, sym)
+            self.cached_matrices = (plan, K, M)
         cm = self._cscmap(plan)
         if cache_matrices:
             # while K and M are reused, so are their CSC sums: each frequency is then one
             # streaming pass A = K_csc - K0^2 M_csc instead of a gather over the COO values
             if self._freq is None or self._freq[0] is not K:
-                self._freq = (K, FreqCsc(cm, K, M, block_symmetric=sym))
+                self._freq = (K, FreqCsc(cm, K, M, block_symmetric=_BLOCKS_SYMMETRIC))
             data = self._freq[1].A(K0)
         else:
-            data = cm.to_csc_combine(K, M, K0, block_symmetric=sym)
+            data = cm.to_csc_combine(K, M, K0, block_symmetric=_BLOCKS_SYMMETRIC)
         return FreqMatrix(data, cm, conductor_tets, fd, K0)
 
     def assemble_scattering_matrix(self, er, ur, tand, cond, frequency: float,
@@ -204,7 +204,7 @@ class VolumeAssembler:
         K, M = plan.assemble_km(er_m, ur_m, self.precision)
         self.assemblies += 1
         cm = self._cscmap(plan)
-        sym = _symmetric(er_m, ur_m)
+        sym = _BLOCKS_SYMMETRIC
         return (cm.to_csc(K, block_symmetric=sym), cm.to_csc(M, block_symmetric=sym), cm,
                 conductor_tets)
 

@@ -429,9 +429,9 @@ class CscMap:
                block_symmetric: bool = False, segment_gather: bool = False) -> torch.Tensor:
         """CSC values of the COO values ``data`` (complex128 or complex64 on device).
 
-        ``block_symmetric``: every 20x20 block of ``data`` is symmetric (the assembly with
-        scalar, diagonal or symmetric tensors); selects the mirrored-read gather, same
-        result, faster. ``segment_gather`` selects the per-entry perm/seg gather instead of
+        ``block_symmetric``: every 20x20 block of ``data`` is symmetric (every block the
+        element kernels write is, whatever the tensors: the reference mirrors i <= j,
+        curlcurl.py:357-371); selects the mirrored-read gather, same result, faster. ``segment_gather`` selects the per-entry perm/seg gather instead of
         the per-column incidence gather (same result)."""
         prec = PREC_FP64 if data.dtype == torch.complex128 else PREC_MIXED
         if data.dtype not in (torch.complex128, torch.complex64) or data.numel() != self.nnz_coo:
@@ -550,8 +550,8 @@ class FreqCsc:
     (assembler.py:471-494, called per frequency from microwave_3d.py:1464-1470); while the
     element matrices are cached the sums of K and M are too. Each entry is then summed as
     ``sum(E) - k0**2 sum(B)`` instead of ``sum(E - k0**2 B)``: the same matrix up to
-    rounding. ``block_symmetric``: the blocks of K and M are symmetric (scalar or diagonal
-    materials), which selects the mirrored column gather.
+    rounding. ``block_symmetric``: the blocks of K and M are symmetric (true of every block
+    the element kernels write), which selects the mirrored column gather.
     """
 
     def __init__(self, cscmap: CscMap, K: torch.Tensor, M: torch.Tensor,

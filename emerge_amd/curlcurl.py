@@ -90,10 +90,10 @@ class CSCMapping:
         return self._lazy("col_offsets")
 
     def to_csc_device(self, data: torch.Tensor) -> torch.Tensor:
-        """CSC values of device COO values. Blocks that are symmetric (scalar or diagonal
-        materials, and any elementwise combination of such blocks, e.g. E - K0**2 B) are
-        summed by the mirrored column gather, others by the staged one; both give the
-        plain gather's result bit for bit (up to the sign of zero sums)."""
+        """CSC values of device COO values. Blocks that are symmetric (the element
+        matrices of any tensors, and elementwise combinations of them such as
+        E - K0**2 B) are summed by the mirrored column gather, others by the staged one;
+        both give the plain gather's result bit for bit (up to the sign of zero sums)."""
         data = data.contiguous()
         if data.numel() == 0:
             return self._dev.to_csc(data)

@@ -88,17 +88,13 @@ class CscSweepGraph:
     ``"mixed"``. The COO blocks and the CSC sums of K and M stay allocated with the graph.
     """
 
-    def __init__(self, plan, cscmap, er, ur, k0s, precision="mixed", out=None,
-                 block_symmetric=None):
+    def __init__(self, plan, cscmap, er, ur, k0s, precision="mixed", out=None):
         import numpy as np
         from .assembly import _out_dtype, _prec, csc_freq_combine
         self.plan = plan
         self.map = cscmap
         self.er = plan.material(er)
         self.ur = plan.material(ur)
-        if block_symmetric is None:
-            block_symmetric = self.er.kind != _lib.EM_MAT_FULL and \
-                self.ur.kind != _lib.EM_MAT_FULL
         self.k0s = np.ascontiguousarray(np.asarray(k0s, np.float64).ravel())
         prec = _prec(precision)
         if prec == _lib.EM_PREC_FP64_REF:
@@ -115,8 +111,10 @@ class CscSweepGraph:
 
         def record():
             plan.assemble_km(self.er, self.ur, precision, out=(self.K, self.M))
-            cscmap.to_csc(self.K, out=self.K_csc, block_symmetric=block_symmetric)
-            cscmap.to_csc(self.M, out=self.M_csc, block_symmetric=block_symmetric)
+            # the element blocks are symmetric whatever the tensors (i <= j mirrored, as
+            # curlcurl.py:357-371): mirrored-read gathers
+            cscmap.to_csc(self.K, out=self.K_csc, block_symmetric=True)
+            cscmap.to_csc(self.M, out=self.M_csc, block_symmetric=True)
             csc_freq_combine(self.K_csc, self.M_csc, self.k0s, out=self.out)
 
         self.graph = Graph(record)

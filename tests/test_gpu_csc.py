@@ -124,10 +124,37 @@ def test_blocks_symmetric_and_mirrored_gather(delaunay_field, kinds, precision):
                        cm.to_csc_combine(K, M, k0, block_symmetric=True))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp64_ref"])
+def test_full_tensor_blocks_bit_symmetric(delaunay_field, precision):
+    """The reference computes each element matrix for i <= j only and mirrors it
+    (curlcurl.py:357-371), so its 20x20 blocks are symmetric whatever the tensors are;
+    the kernels do the same: with full non-symmetric complex eps_r and mu_r every block
+    of K, M, A and of a multi-wavenumber A equals its transpose bit for bit, and the
+    mirrored-read gather returns the plain gather's values exactly."""
+    import torch
+    from emerge_amd.assembly import CscMap
+    rng = np.random.default_rng(24)
+    T = delaunay_field.n_tets
+    er, ur = ref_tensor(T, "full", rng), ref_tensor(T, "full", rng)
+    assert not np.allclose(er, er.transpose(1, 0, 2))
+    plan = _field_plan(delaunay_field)
+    cm = CscMap(plan)
+    k0 = 2 * np.pi * 9.5e9 / 299792458.0
+    K, M = plan.assemble_km(er, ur, precision)
+    A = plan.assemble_a(er, ur, k0, precision)
+    mats = [K, M, A] + list(plan.assemble_a(er, ur, [0.5 * k0, k0, 1.5 * k0], precision))
+    for X in mats:
+        blk = X.view(T, 20, 20)
+        assert torch.equal(blk, blk.transpose(1, 2))
+        assert float(torch.linalg.norm(blk.real)) > 0
+    for X in (K, M, A):
+        assert torch.equal(cm.to_csc(X), cm.to_csc(X, block_symmetric=True))
+
+
 def test_mirrored_gather_symmetric_full_tensors(delaunay_field):
-    """Symmetric full tensors: the blocks are symmetric to rounding (the general path
-    forms G and its antisymmetric part in FP64), and the mirrored gather agrees with the
-    plain one to that level."""
+    """Symmetric full tensors (the full-tensor kernel path, whose antisymmetric metric
+    part is then rounding noise): symmetric blocks, and the mirrored gather agrees with
+    the plain one."""
     import torch
     from emerge_amd.assembly import CscMap
     rng = np.random.default_rng(23)

@@ -267,9 +267,10 @@ def test_volume_assembler_device_inputs(wr90_field, which):
 
 
 def test_volume_assembler_cached_blocks_keep_their_symmetry(wr90_field):
-    """Cached K/M reused with materials of another structure: the CSC combine reads the
-    cached blocks as the materials they were built from require (non-symmetric blocks of
-    a full er are never read mirrored because a later call passes diagonal tensors)."""
+    """Cached K/M built from full tensors, reused by a call with diagonal tensors of
+    another frequency (nothing frequency dependent, same plan): the result is the oracle's
+    A of the materials the blocks were built from, through the mirrored CSC gathers that
+    every element block allows (the blocks are symmetric whatever the tensors)."""
     import oracle
     from emerge_amd.assembler import VolumeAssembler
     f = wr90_field

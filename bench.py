@@ -96,12 +96,21 @@ def rel_err_chunked(a32, a64, chunk=1 << 26) -> float:
 
 
 def tuning_record(r: dict) -> dict:
-    """The autotuner's choice and the fastest time it measured per window."""
-    best = {}
+    """The autotuner's choice and the fastest time it measured per window, without and
+    with prefetched launches."""
+    best, best_pf = {}, {}
     for k, ms in r["ms"].items():
         w = k.split("/")[0]
-        best[w] = min(ms, best.get(w, ms))
-    return {"tile": r["nb"], "window": r["window"], "best_ms_by_window": best}
+        d = best_pf if "/pf" in k else best
+        d[w] = min(ms, d.get(w, ms))
+    return {"tile": r["nb"], "window": r["window"], "prefetch_tets": r.get("prefetch", 0),
+            "best_ms_by_window": best, "best_ms_by_window_prefetched": best_pf}
+
+
+def apply_tuning(plan, rec: dict) -> None:
+    plan.set_window(rec["window"])
+    plan.set_tuning(rec["tile"])
+    plan.set_prefetch(rec["prefetch_tets"])
 
 
 def host_cpu_info() -> dict:
@@ -994,16 +1003,18 @@ def main() -> int:
         for w in (512, 1024, 4096):
             plan.set_window(w)
             plan.set_tuning(128)
-            cold[f"{w}/128"] = cold_cache_launches(
-                lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))
-        plan.set_window(tuning["mixed"]["window"])
-        plan.set_tuning(tuning["mixed"]["tile"])
+            for pf in (0, 1 << 20):
+                plan.set_prefetch(pf)
+                cold[f"{w}/128" + ("/pf" if pf else "")] = cold_cache_launches(
+                    lambda: plan.assemble_a(eps, mu, k0, "mixed", out=a32))
+        apply_tuning(plan, tuning["mixed"])
         cold_best = min(cold.values())
         cold_rec = {"ms_per_launch": cold, "best_ms": cold_best,
                     "roofline_frac_best": T * b32c / (cold_best * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                    "note": "each launch after a 1 GiB device fill (median of 8); the "
-                            "headline times back-to-back launches, whose inputs (~240 MB) "
-                            "stay in the 256 MB memory-side cache"}
+                    "note": "each launch after a 1 GiB device fill (median of 8); '/pf': "
+                            "prefetched launches (chunks of 2^20 tets, each after a kernel "
+                            "reading its inputs); the headline times back-to-back launches, "
+                            "whose inputs (~240 MB) stay in the 256 MB memory-side cache"}
         log(f"mixed, cold cache: {cold}")
     res64 = None
     err = None
@@ -1014,8 +1025,7 @@ def main() -> int:
         ms64, wall64 = time_launches(lambda: plan.assemble_a(eps, mu, k0, "fp64", out=a64),
                                      args.steps, args.warmup)
         log(f"fp64: {ms64:.3f} ms/launch")
-        plan.set_window(tuning["mixed"]["window"])
-        plan.set_tuning(tuning["mixed"]["tile"])
+        apply_tuning(plan, tuning["mixed"])
         plan.assemble_a(eps, mu, k0, "mixed", out=a32)
         torch.cuda.synchronize()
         err = rel_err_chunked(a32, a64)

@@ -80,6 +80,8 @@ _SIGS = {
     "em_ned2_plan_get_tuning": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
     "em_ned2_plan_set_window": (ctypes.c_int, [P, I64]),
     "em_ned2_plan_get_window": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
+    "em_ned2_plan_set_prefetch": (ctypes.c_int, [P, I64]),
+    "em_ned2_plan_get_prefetch": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
     "em_ned2_assemble_csc": (ctypes.c_int, [P, P, ctypes.POINTER(em_material_t),
                                             ctypes.POINTER(em_material_t), ctypes.c_double, I32,
                                             P, I32, P]),

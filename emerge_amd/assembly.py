@@ -248,6 +248,17 @@ class Ned2Plan:
         check(_lib.lib().em_ned2_plan_get_window(self._h, ctypes.byref(v)))
         return v.value
 
+    def set_prefetch(self, chunk_tets: int) -> None:
+        """Prefetched launches: chunks of about ``chunk_tets`` tets, each launched right
+        after a kernel that reads its inputs into the memory-side cache (0: one launch).
+        Same outputs bit for bit; set by the autotuner."""
+        check(_lib.lib().em_ned2_plan_set_prefetch(self._h, int(chunk_tets)))
+
+    def get_prefetch(self) -> int:
+        v = ctypes.c_int64()
+        check(_lib.lib().em_ned2_plan_get_prefetch(self._h, ctypes.byref(v)))
+        return v.value
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.lib().em_ned2_plan_destroy(self._h)

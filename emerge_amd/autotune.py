@@ -1,6 +1,6 @@
 """Autotuner for the assembly kernels' launch shape.
 
-Two knobs, tuned together:
+Three knobs, tuned together:
 
 * tets per workgroup tile (32, 64 or 128): the latency of the per-tile geometry phase and
   the tail of partly filled tiles against occupancy;
@@ -8,9 +8,13 @@ Two knobs, tuned together:
   vertex order and its tiles run on one XCD. Small windows keep the blocks written at the
   same time close together in memory (the kernel is bound by its store stream); windows so
   small that a vertex-order group holds a few tets leave the tiles nearly empty. The best
-  window moves with the mesh size (512 on a 2.8M-tet mesh, 1,024-4,096 on an 8M one).
+  window moves with the mesh size (512 on a 2.8M-tet mesh, 1,024-4,096 on an 8M one);
+* prefetched launches (0 or chunks of 2^20 tets): each chunk of the tile list launched
+  right after a kernel that reads its inputs into the memory-side cache. It pays where
+  the inputs are not already there (a mesh whose inputs exceed the 256 MB cache, or
+  launches after other work) and costs the extra reads where they are.
 
-The best pair depends on the precision, the output mode (one k0 or a sweep), the material
+The best choice depends on the precision, the output mode (one k0 or a sweep), the material
 kinds and the mesh size. ``tune`` times each candidate pair with HIP events on the launch
 stream, applies the fastest to the plan and remembers it in a JSON cache keyed by all of
 the above, so later plans of the same kind skip the timing.
@@ -28,6 +32,7 @@ import torch
 
 CANDIDATES = (32, 64, 128)
 WINDOWS = (512, 1024, 4096, 16384)
+PREFETCH = (0, 1 << 20)
 
 
 def cache_path() -> str:
@@ -98,12 +103,15 @@ def _time(fn, reps: int, warmup: int) -> float:
 
 def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, warmup: int = 2,
          use_cache: bool = True, candidates=CANDIDATES, windows=WINDOWS,
-         rounds: int = 2, cold: bool = False, flush_bytes: int = 1 << 30) -> dict:
-    """Pick the fastest (window, tile) pair for ``plan.assemble_a(er, ur, k0, precision)``.
+         rounds: int = 2, cold: bool = False, flush_bytes: int = 1 << 30,
+         prefetch=PREFETCH) -> dict:
+    """Pick the fastest (window, tile, prefetch) choice for
+    ``plan.assemble_a(er, ur, k0, precision)``.
 
-    Returns ``{"nb": tile, "window": window, "ms": {"window/tile": ms}, "cached": bool,
-    "key": key}`` and leaves the plan set to the best pair. ``windows=None`` keeps the
-    plan's window and tunes the tile only. Every candidate is timed in
+    Returns ``{"nb": tile, "window": window, "prefetch": chunk_tets, "ms":
+    {"window/tile[/pf]": ms}, "cached": bool, "key": key}`` and leaves the plan set to the
+    best choice. ``windows=None`` keeps the plan's window, ``prefetch=None`` the plan's
+    prefetch setting. Every candidate is timed in
     ``rounds`` interleaved passes and keeps its fastest pass: single trials of one
     candidate vary by a few percent, as much as the candidates differ.
 
@@ -119,6 +127,10 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
         windows = (plan.get_window(),)
     else:
         key += "|win"
+    if prefetch is None:
+        prefetch = (plan.get_prefetch(),)
+    else:
+        key += "|pf"
     if cold:
         key += "|cold"
     cache = _load() if use_cache else {}
@@ -126,8 +138,10 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
         c = cache[key]
         plan.set_window(int(c.get("window", plan.get_window())))
         plan.set_tuning(int(c["nb"]))
-        return {"nb": int(c["nb"]), "window": plan.get_window(), "ms": c.get("ms", {}),
-                "cached": True, "key": key}
+        plan.set_prefetch(int(c.get("prefetch", plan.get_prefetch())))
+        return {"nb": int(c["nb"]), "window": plan.get_window(),
+                "prefetch": plan.get_prefetch(), "ms": c.get("ms", {}), "cached": True,
+                "key": key}
     if out is None:
         from .assembly import _out_dtype, _prec
         out = torch.empty((k0a.shape[0], plan.nnz_coo), dtype=_out_dtype(_prec(precision)),
@@ -139,18 +153,22 @@ def tune(plan, er, ur, k0, precision: str = "mixed", out=None, reps: int = 5, wa
     for _ in range(max(1, rounds)):
         for w in windows:
             plan.set_window(w)
-            for nb in candidates:
-                plan.set_tuning(nb)
-                fn = lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target)  # noqa: E731
-                t = _time_cold(fn, reps, junk) if cold else _time(fn, reps, warmup)
-                times[(w, nb)] = min(t, times.get((w, nb), t))
+            for pf in prefetch:
+                plan.set_prefetch(pf)
+                for nb in candidates:
+                    plan.set_tuning(nb)
+                    fn = lambda: plan.assemble_a(er_m, ur_m, k0arg, precision, out=target)  # noqa: E731
+                    t = _time_cold(fn, reps, junk) if cold else _time(fn, reps, warmup)
+                    times[(w, nb, pf)] = min(t, times.get((w, nb, pf), t))
     del junk
-    best_w, best = min(times, key=times.get)
+    best_w, best, best_pf = min(times, key=times.get)
     plan.set_window(best_w)
     plan.set_tuning(best)
-    ms = {f"{w}/{nb}": v for (w, nb), v in times.items()}
+    plan.set_prefetch(best_pf)
+    ms = {f"{w}/{nb}" + (f"/pf{pf}" if pf else ""): v for (w, nb, pf), v in times.items()}
     if use_cache:
         cache = _load()
-        cache[key] = {"nb": best, "window": best_w, "ms": ms}
+        cache[key] = {"nb": best, "window": best_w, "prefetch": best_pf, "ms": ms}
         _save(cache)
-    return {"nb": best, "window": best_w, "ms": ms, "cached": False, "key": key}
+    return {"nb": best, "window": best_w, "prefetch": best_pf, "ms": ms, "cached": False,
+            "key": key}

@@ -42,12 +42,24 @@ struct em_ned2_plan_s {
     int* order = nullptr;           // assembly indices grouped by vertex-rank code
     int2* tiles[3] = {nullptr, nullptr, nullptr};
     long long ntiles[3] = {0, 0, 0};
+    std::vector<int2> htiles[3];    // host copies of the tile lists
+    std::vector<int> htetids;       // host copy of tetids (empty: every tet assembled)
+    // prefetched launches: tets per chunk (0: one launch, no prefetch) and the chunks of
+    // each tile list
+    long long prefetch_tets = 0;
+    std::vector<em::PrefetchChunk> chunks[3];
+    unsigned long long* sink = nullptr;  // the prefetch kernel's never-taken store
 
     void tilesets(em::TileSet ts[3]) const {
         for (int i = 0; i < 3; ++i) {
             ts[i].order = order;
             ts[i].tiles = tiles[i];
             ts[i].ntiles = ntiles[i];
+            const bool pf = prefetch_tets > 0 && !chunks[i].empty();
+            ts[i].chunks = pf ? chunks[i].data() : nullptr;
+            ts[i].nchunks = pf ? (int)chunks[i].size() : 0;
+            ts[i].n_nodes = n_nodes;
+            ts[i].sink = sink;
         }
     }
 
@@ -106,6 +118,7 @@ void free_plan(em_ned2_plan_s* p) {
     (void)hipFree(p->order);
     for (int i = 0; i < 3; ++i) (void)hipFree(p->tiles[i]);
     (void)hipFree(p->csc_rec);
+    (void)hipFree(p->sink);
     delete p;
 }
 
@@ -201,6 +214,54 @@ void build_tiles(const std::vector<unsigned char>& code, long long window, std::
     }
 }
 
+// Chunks of a plan's tile lists for prefetched launches (em_ned2_plan_set_prefetch): about
+// prefetch_tets assembly indices each, cut at whole rounds of the XCD interleave (8 tiles).
+// A tile's tets lie in its window [w W, (w + 1) W) with w = (first order position) / W, so
+// a chunk reads order positions and assembly indices in [min w W, (max w + 1) W) and the
+// tet ids that tetids maps those indices to. Host work only; O(tiles + assembly indices).
+void build_chunks(em_ned2_plan_s* plan) {
+    for (int k = 0; k < 3; ++k) plan->chunks[k].clear();
+    if (plan->prefetch_tets <= 0 || plan->n_assy == 0) return;
+    const long long na = plan->n_assy, W = plan->window;
+    const long long nch = std::min<long long>(4096, (na + plan->prefetch_tets - 1) / plan->prefetch_tets);
+    for (int k = 0; k < 3; ++k) {
+        const std::vector<int2>& tl = plan->htiles[k];
+        const long long rounds = ((long long)tl.size() + 7) / 8;
+        const long long c_n = std::max<long long>(1, std::min(nch, rounds));
+        for (long long c = 0; c < c_n; ++c) {
+            em::PrefetchChunk ch;
+            const long long b = std::min<long long>((long long)tl.size(), 8 * (rounds * c / c_n));
+            const long long e = std::min<long long>((long long)tl.size(), 8 * (rounds * (c + 1) / c_n));
+            ch.tile_begin = b;
+            ch.tile_count = e - b;
+            long long wlo = -1, whi = -1;
+            for (long long i = b; i < e; ++i) {
+                if ((tl[i].y & 0xffff) == 0) continue;  // padding tile
+                const long long w = tl[i].x / W;
+                wlo = wlo < 0 ? w : std::min(wlo, w);
+                whi = std::max(whi, w);
+            }
+            if (wlo >= 0) {
+                ch.ia_lo = wlo * W;
+                ch.ia_hi = std::min(na, (whi + 1) * W);
+                if (plan->htetids.empty()) {
+                    ch.t_lo = ch.ia_lo;
+                    ch.t_hi = ch.ia_hi;
+                } else {
+                    int lo = plan->htetids[ch.ia_lo], hi = lo;
+                    for (long long i = ch.ia_lo; i < ch.ia_hi; ++i) {
+                        lo = std::min(lo, plan->htetids[i]);
+                        hi = std::max(hi, plan->htetids[i]);
+                    }
+                    ch.t_lo = lo;
+                    ch.t_hi = (long long)hi + 1;
+                }
+            }
+            plan->chunks[k].push_back(ch);
+        }
+    }
+}
+
 // Rebuild a plan's tile lists for another window: upload, wait for launches still
 // reading the old lists, swap.
 int rebuild_tiles(em_ned2_plan_s* plan, long long window) {
@@ -238,8 +299,10 @@ int rebuild_tiles(em_ned2_plan_s* plan, long long window) {
         (void)hipFree(plan->tiles[k]);
         plan->tiles[k] = tiles[k];
         plan->ntiles[k] = (long long)htiles[k].size();
+        plan->htiles[k] = std::move(htiles[k]);
     }
     plan->window = window;
+    build_chunks(plan);
     return EM_OK;
 }
 
@@ -358,7 +421,7 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         (st = dmalloc(&p->t2f, ht2f.size())) || (st = dmalloc(&p->tetids, hids.size())) ||
         (st = dmalloc(&p->order, horder.size())) || (st = dmalloc(&p->tiles[0], htiles[0].size())) ||
         (st = dmalloc(&p->tiles[1], htiles[1].size())) ||
-        (st = dmalloc(&p->tiles[2], htiles[2].size()))) {
+        (st = dmalloc(&p->tiles[2], htiles[2].size())) || (st = dmalloc(&p->sink, 1))) {
         free_plan(p);
         return st;
     }
@@ -431,6 +494,8 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
         return EM_EDEGENERATE;
     }
     p->code = std::move(code);
+    for (int k = 0; k < 3; ++k) p->htiles[k] = std::move(htiles[k]);
+    p->htetids = std::move(hids);
     *plan = p;
     return EM_OK;
 }
@@ -645,6 +710,29 @@ int em_ned2_plan_set_window(em_ned2_plan_t plan, int64_t window) {
     }
     if (window == plan->window) return EM_OK;
     return rebuild_tiles(plan, window);
+}
+
+int em_ned2_plan_set_prefetch(em_ned2_plan_t plan, int64_t chunk_tets) {
+    if (!plan) {
+        set_error("null plan");
+        return EM_EINVAL;
+    }
+    if (chunk_tets != 0 && (chunk_tets < 4096 || chunk_tets > (1ll << 31))) {
+        set_error("prefetch: the chunk must hold 4,096 to 2^31 tets, or 0 (off)");
+        return EM_EINVAL;
+    }
+    plan->prefetch_tets = chunk_tets;
+    build_chunks(plan);
+    return EM_OK;
+}
+
+int em_ned2_plan_get_prefetch(em_ned2_plan_t plan, int64_t* chunk_tets) {
+    if (!plan || !chunk_tets) {
+        set_error("null argument");
+        return EM_EINVAL;
+    }
+    *chunk_tets = plan->prefetch_tets;
+    return EM_OK;
 }
 
 int em_ned2_plan_get_window(em_ned2_plan_t plan, int64_t* window) {

@@ -50,10 +50,25 @@ struct K0Pack {
 // assembly indices window by window, bucket by bucket (ascending within a bucket);
 // tiles[] = (first position in order, count | code << 16) with at most `tile` tets each,
 // interleaved so that all tiles of a window land on one XCD; count 0 marks a padding tile.
+// A slice of a tile list that is launched right after a prefetch kernel has read the
+// inputs its tiles use: tiles [tile_begin, tile_begin + tile_count), assembly indices (and
+// order positions) [ia_lo, ia_hi), tet ids [t_lo, t_hi).
+struct PrefetchChunk {
+    long long tile_begin = 0, tile_count = 0;
+    long long ia_lo = 0, ia_hi = 0;
+    long long t_lo = 0, t_hi = 0;
+};
+
 struct TileSet {
     const int* order = nullptr;
     const int2* tiles = nullptr;
     long long ntiles = 0;
+    // Host array of nchunks slices covering the list in order, each launched after a
+    // prefetch of its inputs; nullptr: one launch over the whole list, no prefetch.
+    const PrefetchChunk* chunks = nullptr;
+    int nchunks = 0;
+    long long n_nodes = 0;             // node coordinates, read by the first prefetch
+    unsigned long long* sink = nullptr;  // device word for the prefetch's never-taken store
 };
 constexpr int kTileSizes[3] = {32, 64, 128};
 

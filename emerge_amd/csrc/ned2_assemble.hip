@@ -500,8 +500,77 @@ __global__ __launch_bounds__(kBlock) void ned2_assemble_kernel(
 #undef EM_STREAM
 }
 
+// Folds ld(k) for k in [lo, hi) (grid-stride from i0) into one word, four loads in
+// flight per thread: indices past hi load element hi - 1 (in bounds, as k0 < hi) and are
+// masked out, so the loads need no branch and no wait between them.
+template <typename Load>
+__device__ __forceinline__ unsigned long long fold4(long long lo, long long hi, long long i0,
+                                                   long long stride, Load ld) {
+    unsigned long long x = 0;
+    for (long long k0 = lo + i0; k0 < hi; k0 += 4 * stride) {
+        unsigned long long v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long k = k0 + u * stride;
+            v[u] = ld(min(k, hi - 1)) & (k < hi ? ~0ull : 0ull);
+        }
+        x ^= v[0] ^ v[1] ^ v[2] ^ v[3];
+    }
+    return x;
+}
+
+// Material values of tets [t0, t1) as the assembly kernel reads them (load_tensor).
+__device__ __forceinline__ unsigned long long touch_material(const MatDesc& d, long long t0,
+                                                            long long t1, long long i0,
+                                                            long long stride) {
+    unsigned long long x = 0;
+    const int nc = d.kind == EM_MAT_SCALAR ? 1 : d.kind == EM_MAT_DIAG ? 3 : 9;
+    if (d.ts == 0) t1 = min(t1, t0 + 1);  // one tensor for every tet
+    for (int c = 0; c < nc; ++c) {
+        const double2* p = d.p + (long long)c * d.cs;
+        const long long ts = d.ts;
+        x ^= fold4(t0, t1, i0, stride, [=](long long t) {
+            return (unsigned long long)__double_as_longlong(p[t * ts].x);
+        });
+    }
+    return x;
+}
+
+// Reads the inputs of one chunk of tiles so that the assembly launch right after it
+// finds them in the memory-side cache instead of DRAM: the node coordinates (first chunk
+// only), order positions and tetids [ia_lo, ia_hi), tet node ids and material values of
+// tets [t_lo, t_hi). Grid-stride; the values are folded into one word that is stored
+// only if it equals `key`, which keeps the loads without writing anything in practice.
+__global__ __launch_bounds__(256) void prefetch_chunk_kernel(MeshDev m, long long n_nodes,
+                                                             const int* __restrict__ order,
+                                                             PrefetchChunk ch, MatDesc eps,
+                                                             MatDesc mu, unsigned long long key,
+                                                             unsigned long long* sink) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const double4* xyz = m.xyz;
+    const int4* tets = m.tets;
+    const int* ids = m.tetids;
+    unsigned long long x = fold4(0, n_nodes, i0, stride, [=](long long j) {
+        return (unsigned long long)__double_as_longlong(xyz[j].x);
+    });
+    x ^= fold4(ch.ia_lo, ch.ia_hi, i0, stride,
+               [=](long long j) { return (unsigned long long)(unsigned)order[j]; });
+    if (ids)
+        x ^= fold4(ch.ia_lo, ch.ia_hi, i0, stride,
+                   [=](long long j) { return (unsigned long long)(unsigned)ids[j] << 32; });
+    x ^= fold4(ch.t_lo, ch.t_hi, i0, stride,
+               [=](long long j) { return (unsigned long long)(unsigned)tets[j].x; });
+    x ^= touch_material(eps, ch.t_lo, ch.t_hi, i0, stride);
+    x ^= touch_material(mu, ch.t_lo, ch.t_hi, i0, stride) << 1;
+    if (x == key) *sink = x;
+}
+
 // Launch over all tiles; MODE_AF launches once per chunk of up to kMaxK0 wavenumbers
 // (passed by value, so the launch needs no device buffer and can be graph-captured).
+// With prefetch chunks (ts.chunks) each slice of the tile list is its own launch, preceded
+// by prefetch_chunk_kernel over the inputs of its tiles; every tile computes exactly what
+// it computes in the single launch, so the outputs are the same bit for bit.
 template <typename R, int MODE, bool FULL, int TILE>
 static hipError_t launch_t(const MeshDev& m, const TileSet& ts, const MatDesc& eps,
                            const MatDesc& mu, const double* k0sq, int nf, void* out0, void* out1,
@@ -514,11 +583,31 @@ static hipError_t launch_t(const MeshDev& m, const TileSet& ts, const MatDesc& e
         const int n = (MODE == MODE_AF) ? min(kMaxK0, nf - f0) : 1;
         for (int f = 0; f < kMaxK0; ++f) p.v[f] = (k0sq && f < n) ? k0sq[f0 + f] : 0.0;
         void* o0 = reinterpret_cast<O*>(out0) + (long long)f0 * freq_stride;
-        hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, FULL, TILE>), dim3((unsigned)ts.ntiles),
-                           dim3(kBlock), 0, stream, m, eps, mu, p, n, ts.order, ts.tiles, o0, out1,
-                           freq_stride);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (!ts.chunks) {
+            hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, FULL, TILE>),
+                               dim3((unsigned)ts.ntiles), dim3(kBlock), 0, stream, m, eps, mu, p,
+                               n, ts.order, ts.tiles, o0, out1, freq_stride);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            continue;
+        }
+        for (int c = 0; c < ts.nchunks; ++c) {
+            const PrefetchChunk& ch = ts.chunks[c];
+            if (ch.tile_count == 0) continue;
+            const long long span = max(max(ch.t_hi - ch.t_lo, ch.ia_hi - ch.ia_lo),
+                                       c == 0 ? ts.n_nodes : 0ll);
+            // four elements per thread and array: every load of the chunk in flight at once
+            const unsigned pgrid = (unsigned)min(8192ll, max(1ll, (span + 1023) / 1024));
+            hipLaunchKernelGGL(prefetch_chunk_kernel, dim3(pgrid), dim3(256), 0, stream, m,
+                               c == 0 ? ts.n_nodes : 0ll, ts.order, ch, eps, mu,
+                               0x9e3779b97f4a7c15ull, ts.sink);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((ned2_assemble_kernel<R, MODE, FULL, TILE>),
+                               dim3((unsigned)ch.tile_count), dim3(kBlock), 0, stream, m, eps, mu,
+                               p, n, ts.order, ts.tiles + ch.tile_begin, o0, out1, freq_stride);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }

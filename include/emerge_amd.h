@@ -159,6 +159,18 @@ int em_ned2_plan_get_tuning(em_ned2_plan_t plan, int32_t* tile);
  */
 int em_ned2_plan_set_window(em_ned2_plan_t plan, int64_t window);
 int em_ned2_plan_get_window(em_ned2_plan_t plan, int64_t* window);
+/*
+ * A third launch-shape knob (no counterpart in the reference): prefetched launches. With
+ * chunk_tets > 0 (4,096 .. 2^31) em_ned2_assemble_km / em_ned2_assemble_a split the tile
+ * list into chunks of about chunk_tets tets; each chunk is one launch, right after a small
+ * kernel that reads the node coordinates (first chunk), tet ids and material values its
+ * tiles use, so that the assembly finds them in the memory-side cache instead of DRAM.
+ * Worth it when the inputs are not already cached: meshes whose inputs exceed the cache,
+ * or launches after other work. Same outputs bit for bit; 0 (the default) is one launch.
+ * Host-side only (no device work, no synchronisation); set by the autotuner.
+ */
+int em_ned2_plan_set_prefetch(em_ned2_plan_t plan, int64_t chunk_tets);
+int em_ned2_plan_get_prefetch(em_ned2_plan_t plan, int64_t* chunk_tets);
 
 /*
  * Element stiffness and mass blocks, reference-style (curlcurl.py:387-456):

@@ -48,13 +48,16 @@ def test_autotuner_picks_and_caches(delaunay_field, tmp_path, monkeypatch):
     k0 = 2 * np.pi * 10e9 / C0
     r = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, warmup=1)
     assert r["nb"] in (32, 64, 128) and r["window"] in autotune.WINDOWS and not r["cached"]
-    assert set(r["ms"]) == {f"{w}/{nb}" for w in autotune.WINDOWS for nb in (32, 64, 128)}
+    assert set(r["ms"]) == {f"{w}/{nb}" + (f"/pf{pf}" if pf else "") for w in autotune.WINDOWS
+                            for nb in (32, 64, 128) for pf in autotune.PREFETCH}
+    assert r["prefetch"] in autotune.PREFETCH
     assert plan.get_tuning() == r["nb"] and plan.get_window() == r["window"]
+    assert plan.get_prefetch() == r["prefetch"]
     plan2 = Ned2Plan.from_field(delaunay_field)
-    assert plan2.get_window() == 1024
+    assert plan2.get_window() == 1024 and plan2.get_prefetch() == 0
     r2 = autotune.tune(plan2, eps, mu, k0, "mixed")
     assert r2["cached"] and r2["nb"] == r["nb"] and plan2.get_tuning() == r["nb"]
-    assert plan2.get_window() == r["window"]
+    assert plan2.get_window() == r["window"] and plan2.get_prefetch() == r["prefetch"]
 
 
 def test_autotuner_cold_cache_mode(delaunay_field, tmp_path, monkeypatch):
@@ -69,9 +72,11 @@ def test_autotuner_cold_cache_mode(delaunay_field, tmp_path, monkeypatch):
     k0 = 2 * np.pi * 10e9 / C0
     ref = plan.assemble_a(eps, mu, k0, "mixed").cpu()
     r = autotune.tune(plan, eps, mu, k0, "mixed", reps=2, cold=True, flush_bytes=1 << 24,
-                      windows=(512, 1024), candidates=(64, 128), rounds=1)
+                      windows=(512, 1024), candidates=(64, 128), rounds=1,
+                      prefetch=(0, 4096))
     assert r["key"].endswith("|cold") and not r["cached"]
-    assert set(r["ms"]) == {"512/64", "512/128", "1024/64", "1024/128"}
+    assert set(r["ms"]) == {f"{w}/{nb}" + p for w in (512, 1024) for nb in (64, 128)
+                            for p in ("", "/pf4096")}
     assert all(v > 0 for v in r["ms"].values())
     assert plan.get_tuning() == r["nb"] and plan.get_window() == r["window"]
     assert torch.equal(plan.assemble_a(eps, mu, k0, "mixed").cpu(), ref)
@@ -104,6 +109,53 @@ def test_window_variants_identical(delaunay_field, precision):
             assert torch.equal(A, ref), (w, tile)
     with pytest.raises(Exception):
         plan.set_window(16)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_prefetched_launches_identical(delaunay_field, precision):
+    """Prefetched launches (the tile list cut into chunks, each launched after a kernel
+    that reads its inputs) give K, M, A and a three-wavenumber A equal bit for bit to one
+    launch: several chunk sizes (one chunk per 4,096 tets up to one for the whole mesh),
+    windows and tiles, per-tet and (3,3,T) full tensors, conductor tets removed."""
+    import torch
+    from emerge_amd.assembly import Ned2Plan
+    T = delaunay_field.n_tets
+    rng = np.random.default_rng(9)
+    keep = np.sort(rng.choice(T, size=T - T // 7, replace=False))
+    k0 = 2 * np.pi * 9e9 / C0
+    k0s = [0.7 * k0, k0, 1.3 * k0]
+    eps, mu = _mats(T, rng)
+    er_full, ur_full = ref_tensor(T, "full", rng), ref_tensor(T, "full", rng)
+    for tetids in (None, keep):
+        plan = Ned2Plan.from_field(delaunay_field, tetids=tetids)
+        assert plan.get_prefetch() == 0
+        for er, ur in ((eps, mu), (er_full, ur_full)):
+            for w, tile in ((1024, 128), (512, 32)):
+                plan.set_window(w)
+                plan.set_tuning(tile)
+                plan.set_prefetch(0)
+                ref = [plan.assemble_a(er, ur, k0, precision).cpu(),
+                       *[x.cpu() for x in plan.assemble_km(er, ur, precision)],
+                       plan.assemble_a(er, ur, k0s, precision).cpu()]
+                for chunk in (4096, 6000, 1 << 20):
+                    plan.set_prefetch(chunk)
+                    assert plan.get_prefetch() == chunk
+                    got = [plan.assemble_a(er, ur, k0, precision).cpu(),
+                           *[x.cpu() for x in plan.assemble_km(er, ur, precision)],
+                           plan.assemble_a(er, ur, k0s, precision).cpu()]
+                    for a, b in zip(got, ref):
+                        assert torch.equal(a, b), (tetids is None, w, tile, chunk)
+        # the chunks follow a window change
+        plan.set_prefetch(4096)
+        plan.set_window(2048)
+        plan.set_prefetch(0)
+        A0 = plan.assemble_a(eps, mu, k0, precision).cpu()
+        plan.set_prefetch(4096)
+        assert torch.equal(plan.assemble_a(eps, mu, k0, precision).cpu(), A0)
+        for bad in (-1, 1, 4095):
+            with pytest.raises(Exception):
+                plan.set_prefetch(bad)
+        assert plan.get_prefetch() == 4096
 
 
 def test_uneven_multimesh_batch_one_launch():

@@ -191,6 +191,10 @@ def test_uneven_multimesh_batch_one_launch():
         assert rel_fro(A32[sl].astype(complex), dE - k0 ** 2 * dB) <= 1e-6, i
         assert np.array_equal(rows[sl] - plan.dof_offsets[i], r)
         assert np.array_equal(cols[sl] - plan.dof_offsets[i], c)
+    # prefetched launches over the batch (chunks spanning mesh boundaries): the same bits
+    plan.set_prefetch(4096)
+    assert np.array_equal(plan.assemble_a(eps, mu, k0, "fp64").cpu().numpy(), A)
+    assert np.array_equal(plan.assemble_a(eps, mu, k0, "mixed").cpu().numpy(), A32)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -217,6 +221,17 @@ def test_sweep_graph_replays_match_eager(delaunay_field, precision):
     for f in (0, 31, 63):
         one = plan.assemble_a(eps, mu, float(k0s[f]), precision)
         assert float(torch.linalg.norm(one - eager[f]) / torch.linalg.norm(one)) <= tol
+    # with prefetched launches the captured graph holds a prefetch and an assembly launch
+    # per chunk; its replays equal the single-launch sweep bit for bit
+    plan.set_prefetch(4096)
+    sgp = SweepGraph(plan, eps, mu, k0s, precision)
+    for rep in range(3):
+        sgp.out.zero_()
+        junk = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        assert torch.equal(sgp.replay(), eager), rep
+        torch.cuda.synchronize()
+        assert float(junk) == float(junk)
 
 
 def test_sweep_more_than_128_frequencies(delaunay_field):
@@ -232,3 +247,6 @@ def test_sweep_more_than_128_frequencies(delaunay_field):
     for fi in (0, 127, 128, 255, 256, 299):
         one = plan.assemble_a(eps, mu, float(k0s[fi]), "fp64")
         assert float(torch.linalg.norm(out[fi] - one) / torch.linalg.norm(one)) <= 1e-14
+    # prefetched launches inside every wavenumber pack: the same values bit for bit
+    plan.set_prefetch(4096)
+    assert torch.equal(plan.assemble_a(eps, mu, k0s, "fp64"), out)

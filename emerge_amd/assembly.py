@@ -251,7 +251,8 @@ class Ned2Plan:
     def set_prefetch(self, chunk_tets: int) -> None:
         """Prefetched launches: chunks of about ``chunk_tets`` tets, each launched right
         after a kernel that reads its inputs into the memory-side cache (0: one launch).
-        Same outputs bit for bit; set by the autotuner."""
+        Same outputs bit for bit; set by the autotuner. A new plan starts with 2**20 if it
+        assembles at least 2**21 tets, else 0."""
         check(_lib.lib().em_ned2_plan_set_prefetch(self._h, int(chunk_tets)))
 
     def get_prefetch(self) -> int:

@@ -214,6 +214,9 @@ void build_tiles(const std::vector<unsigned char>& code, long long window, std::
     }
 }
 
+constexpr long long kPrefetchDefaultMinTets = 1ll << 21;
+constexpr long long kPrefetchDefaultChunk = 1ll << 20;
+
 // Chunks of a plan's tile lists for prefetched launches (em_ned2_plan_set_prefetch): about
 // prefetch_tets assembly indices each, cut at whole rounds of the XCD interleave (8 tiles).
 // A tile's tets lie in its window [w W, (w + 1) W) with w = (first order position) / W, so
@@ -496,6 +499,13 @@ int em_ned2_plan_create(em_ned2_plan_t* plan, int device, const double* nodes, i
     p->code = std::move(code);
     for (int k = 0; k < 3; ++k) p->htiles[k] = std::move(htiles[k]);
     p->htetids = std::move(hids);
+    // Prefetched launches by default on meshes whose inputs approach the memory-side
+    // cache (2^21 tets and up): there they gain or cost nothing warm and gain cold; on
+    // small meshes the extra launch would cost more than the reads it saves.
+    if (n_assy >= kPrefetchDefaultMinTets) {
+        p->prefetch_tets = kPrefetchDefaultChunk;
+        build_chunks(p);
+    }
     *plan = p;
     return EM_OK;
 }

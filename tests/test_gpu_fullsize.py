@@ -69,12 +69,17 @@ def test_bench_mesh_full_matrix(bench_mesh):
     T = f.n_tets
     assert T == 2_815_239
     plan = Ned2Plan.from_field(f)
+    # a plan of this size starts with prefetched launches (chunks of 2^20 tets)
+    assert plan.get_prefetch() == 1 << 20
     e, m = plan.material(torch.from_numpy(eps)), plan.material(torch.from_numpy(mu))
     a64 = plan.assemble_a(e, m, K0, "fp64")
     a32 = plan.assemble_a(e, m, K0, "mixed")
     err = _rel_dev(a32, a64)
     print(f"mixed vs fp64 over {a64.numel()} entries: {err:.3e}", flush=True)
     assert err <= 1e-6
+    plan.set_prefetch(0)
+    assert torch.equal(plan.assemble_a(e, m, K0, "mixed"), a32)  # one launch: same bits
+    plan.set_prefetch(1 << 20)
     del a32
     cm = CscMap(plan)
     csc = cm.to_csc(a64, block_symmetric=True)

@@ -8,7 +8,8 @@ Three knobs, tuned together:
   vertex order and its tiles run on one XCD. Small windows keep the blocks written at the
   same time close together in memory (the kernel is bound by its store stream); windows so
   small that a vertex-order group holds a few tets leave the tiles nearly empty. The best
-  window moves with the mesh size (512 on a 2.8M-tet mesh, 1,024-4,096 on an 8M one);
+  window moves with the mesh size and the other knobs (512-704 on a 2.8M-tet mesh; on an
+  8M one 1,024-4,096 as one launch, 640-704 prefetched);
 * prefetched launches (0 or chunks of 2^20 tets): each chunk of the tile list launched
   right after a kernel that reads its inputs into the memory-side cache. It pays where
   the inputs are not already there (a mesh whose inputs exceed the 256 MB cache, or
@@ -31,7 +32,7 @@ import numpy as np
 import torch
 
 CANDIDATES = (32, 64, 128)
-WINDOWS = (512, 1024, 4096, 16384)
+WINDOWS = (512, 640, 768, 1024, 4096, 16384)
 PREFETCH = (0, 1 << 20)
 
 

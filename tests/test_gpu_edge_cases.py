@@ -115,6 +115,9 @@ def test_plan_without_assembled_tets(wr90_field):
     assert K.numel() == 0 and M.numel() == 0
     assert plan.assemble_a(er, ur, K0, "mixed").numel() == 0
     assert tuple(plan.assemble_a(er, ur, np.array([K0, 2 * K0]), "fp64").shape) == (2, 0)
+    plan.set_prefetch(4096)  # prefetched launches of nothing: no chunks, no launch
+    assert plan.assemble_a(er, ur, K0, "mixed").numel() == 0
+    assert all(x.numel() == 0 for x in plan.assemble_km(er, ur, "fp64"))
     r, c = plan.rowcol()
     assert r.numel() == 0 and c.numel() == 0
     cm = CscMap(plan)
